@@ -1,0 +1,158 @@
+"""ctypes wrapper of the CPU oracle (``oracle/pc_oracle.c``).
+
+TEST INFRASTRUCTURE: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module, and only as the
+checker.  The product (``libpcmx``) never loads it.
+
+Arrays follow the reference f-array: ``f(mx,my,mz,nvar)`` column-major, which
+is a C-ordered numpy array of shape ``(nvar, mz, my, mx)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libpcoracle.so")
+NGHOST = 3
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("lmagnetic", ctypes.c_int), ("lentropy", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("chi", ctypes.c_double),
+        ("cs0", ctypes.c_double), ("gamma", ctypes.c_double), ("cp", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("mu0", ctypes.c_double),
+        ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
+        ("dt", ctypes.c_double),
+    ]
+
+
+def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
+                cs0=1.0, gamma=1.0, cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25,
+                dt=1e-3, L=2 * math.pi):
+    """Run parameters with the reference defaults (cdata.f90:235-239,
+    eos_idealgas.f90:40-46)."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    return Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
+                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        pp = ctypes.POINTER(Params)
+        L.pco_nvar.argtypes = [pp]
+        L.pco_iss.argtypes = [pp]
+        L.pco_iax.argtypes = [pp]
+        L.pco_eos_constants.argtypes = [pp, dp]
+        L.pco_fill_ghosts.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        for name in ("pco_der", "pco_der2"):
+            getattr(L, name).argtypes = [dp, pp, ctypes.c_int, ctypes.c_int, dp]
+        L.pco_derij.argtypes = [dp, pp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.pco_pde.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, ctypes.c_int]
+        L.pco_advance.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
+        for name in ("pco_der", "pco_der2", "pco_derij", "pco_fill_ghosts", "pco_pde",
+                     "pco_advance", "pco_eos_constants"):
+            getattr(L, name).restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def nvar(p):
+    return lib().pco_nvar(ctypes.byref(p))
+
+
+def iss(p):
+    return lib().pco_iss(ctypes.byref(p))
+
+
+def iax(p):
+    return lib().pco_iax(ctypes.byref(p))
+
+
+def eos_constants(p):
+    out = np.zeros(8)
+    lib().pco_eos_constants(ctypes.byref(p), _ptr(out))
+    keys = ("cs20", "gamma_m1", "gamma1", "cp1", "cv", "cv1", "lnrho0", "lnTT0")
+    return dict(zip(keys, out.tolist()))
+
+
+def with_ghosts(interior):
+    """(nvar, nz, ny, nx) interior -> (nvar, mz, my, mx) with periodic ghosts."""
+    interior = np.asarray(interior, dtype=np.float64)
+    nv, nz, ny, nx = interior.shape
+    g = NGHOST
+    f = np.zeros((nv, nz + 2 * g, ny + 2 * g, nx + 2 * g))
+    f[:, g:g + nz, g:g + ny, g:g + nx] = interior
+    lib().pco_fill_ghosts(_ptr(f), nx, ny, nz, nv)
+    return f
+
+
+def interior(f):
+    g = NGHOST
+    return f[:, g:-g, g:-g, g:-g]
+
+
+def der(f, p, k, j):
+    out = np.zeros((p.nz, p.ny, p.nx))
+    lib().pco_der(_ptr(f), ctypes.byref(p), k, j, _ptr(out))
+    return out
+
+
+def der2(f, p, k, j):
+    out = np.zeros((p.nz, p.ny, p.nx))
+    lib().pco_der2(_ptr(f), ctypes.byref(p), k, j, _ptr(out))
+    return out
+
+
+def derij(f, p, k, i, j):
+    out = np.zeros((p.nz, p.ny, p.nx))
+    lib().pco_derij(_ptr(f), ctypes.byref(p), k, i, j, _ptr(out))
+    return out
+
+
+def rhs(interior_f, p, want_dt=False, nthreads=0):
+    """One RHS evaluation (df starting at zero).  Returns (rhs_interior,
+    dt1_max, (urms, umax, brms))."""
+    f = with_ghosts(interior_f)
+    df = np.zeros_like(f)
+    dt1 = np.zeros(1)
+    diag = np.zeros(3)
+    lib().pco_pde(_ptr(f), _ptr(df), ctypes.byref(p), int(want_dt), _ptr(dt1), _ptr(diag), nthreads)
+    return interior(df).copy(), float(dt1[0]), tuple(diag.tolist())
+
+
+def advance(interior_f, p, nsteps, t=0.0, dt=0.0, diag=False, nthreads=0):
+    """nsteps full RK3 steps.  Returns (f_interior, t, dt, diag_series)."""
+    f = with_ghosts(interior_f)
+    df = np.zeros_like(f)
+    tt = ctypes.c_double(t)
+    dd = ctypes.c_double(dt if p.dt == 0.0 else p.dt)
+    series = np.zeros((max(nsteps, 1), 3)) if diag else None
+    lib().pco_advance(_ptr(f), _ptr(df), ctypes.byref(p), nsteps, ctypes.byref(tt), ctypes.byref(dd),
+                      _ptr(series) if diag else None, nthreads)
+    return interior(f).copy(), tt.value, dd.value, series

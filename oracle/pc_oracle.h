@@ -1,0 +1,89 @@
+/*
+ * pc_oracle.h -- CPU restatement of the Pencil Code RHS + 2N-RK3 hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle and the
+ * CPU baseline; it is never linked into, loaded by, or called from the
+ * product library (libpcmx).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may use it, and only as the checker.
+ *
+ * It follows the reference's CPU pencil loop function by function (file:line
+ * citations are given at each routine in pc_oracle.c):
+ *   src/deriv.f90      der_main, der2_main, derij_main (bidiagonal)
+ *   src/sub.f90        gij, div_mn, curl_mn, traceless_strain, multmv_mn,
+ *                      dot_mn, dot2_mn, cross_mn, multm2_sym_mn, del2v_etc,
+ *                      gij_etc, set_dt
+ *   src/hydro.f90, density.f90, viscosity.f90, noentropy.f90, entropy.f90,
+ *   eos_idealgas.f90, magnetic.f90  (pencils and d*_dt RHS terms)
+ *   src/equ.f90        rhs_cpu pencil loop, calc_maxadvec, set_dt1_max
+ *   src/timestep.f90   advance_substeps (Williamson 2N-RK3)
+ *   src/boundcond.f90  bc_per_x/y/z (periodic ghost fill)
+ *   src/diagnostics.f90 urms / umax / brms normalisation
+ *
+ * Memory layout is the reference f-array: f(mx,my,mz,nvar), column-major,
+ * x fastest, mx = nx + 2*NGHOST.  Variable slots follow registration order
+ * (src/register.f90:137-146): uu = 0..2, lnrho = 3, ss = 4 if entropy,
+ * aa = 4..6 (isothermal) or 5..7 (entropy).  Indices here are 0-based.
+ *
+ * Parity status of each function is recorded in the module header of
+ * pc_oracle.c.
+ */
+#ifndef PC_ORACLE_H
+#define PC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCO_NGHOST 3
+
+typedef struct {
+  int nx, ny, nz;          /* interior grid */
+  int lmagnetic;           /* MAGNETIC=magnetic */
+  int lentropy;            /* ENTROPY=entropy (else noentropy) */
+  double Lx, Ly, Lz;       /* box size (default 2*pi, start.f90 xyz0=-pi) */
+  double nu, eta, chi;     /* nu-const viscosity, eta-const resistivity, chi-const conduction */
+  double cs0, gamma, cp, rho0, mu0;
+  double cdt, cdtv;        /* Courant factors (cdata.f90:235-239) */
+  double dt;               /* fixed dt; 0 => Courant-limited (timestep.f90:67) */
+} pco_params;
+
+/* Variable slot helpers (0-based). */
+int pco_nvar(const pco_params* p);
+int pco_iss(const pco_params* p);   /* -1 when no entropy */
+int pco_iax(const pco_params* p);   /* -1 when no magnetic */
+
+/* Derived EOS constants as eos_idealgas.f90:initialize_eos computes them
+ * (cs20, gamma_m1, gamma1, cp1, cv, cv1, lnrho0, lnTT0), in that order. */
+void pco_eos_constants(const pco_params* p, double out[8]);
+
+/* Periodic ghost fill of every variable, boundconds_x then _y then _z over
+ * full extents (boundcond.f90:2387-2433, 2509-). */
+void pco_fill_ghosts(double* f, int nx, int ny, int nz, int nvar);
+
+/* Single stencil applications on a ghost-filled f for all interior points.
+ * out has nx*ny*nz entries, x fastest.  k is the 0-based variable. */
+void pco_der(const double* f, const pco_params* p, int k, int j, double* out);
+void pco_der2(const double* f, const pco_params* p, int k, int j, double* out);
+void pco_derij(const double* f, const pco_params* p, int k, int i, int j, double* out);
+
+/* One pde() call: ghosts are filled, then for every pencil the module RHS
+ * terms are added to df (which the caller pre-scales, as advance_substeps
+ * does), in calc_all_rhs order.  When want_dt, dt1_max (the maximum of the
+ * inverse Courant time step over the domain) is returned in *dt1.
+ * When diag != NULL it receives {urms, umax, brms} of the input state.
+ * nthreads <= 0 uses the OpenMP default. */
+void pco_pde(double* f, double* df, const pco_params* p, int want_dt,
+             double* dt1, double* diag, int nthreads);
+
+/* nsteps full time steps (time_step / advance_substeps).  f and df are
+ * full (mx,my,mz,nvar) arrays.  *dt is the fixed step, or on return the last
+ * Courant step when p->dt == 0.  *t is advanced as the reference does.
+ * diag_series (may be NULL) receives 3 values {urms, umax, brms} per step,
+ * evaluated on the first substep (equ.f90:120). */
+void pco_advance(double* f, double* df, const pco_params* p, int nsteps,
+                 double* t, double* dt, double* diag_series, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
