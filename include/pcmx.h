@@ -1,0 +1,148 @@
+/*
+ * pcmx.h -- C ABI of the MI355X RHS + RK3 engine (libpcmx.so).
+ *
+ * This is the drop-in boundary that a Pencil Code `module GPU`
+ * (GPU = gpu_mi355x in Makefile.local) binds through iso_c_binding.  Each
+ * entry point replaces one C function of the reference's Astaroth plugin
+ * (src/gpu_astaroth_ansi.c, called from src/gpu_astaroth.f90); the citation
+ * is given at each declaration.  Arguments are plain C: ints, doubles and
+ * pointers, no device or torch types.  Every function returns PCMX_OK (0)
+ * or a negative PCMX_ERR_* code; pcmx_last_error() then holds a message
+ * (the reference prints and exits instead, gpu_astaroth.cc:2074-2078; the
+ * Fortran side turns a negative code into fatal_error).
+ *
+ * State layout on the host (the reference f-array, src/cparam.f90:57-64):
+ *   f(mx,my,mz,mfarray) real(8), column-major, x fastest, mx = nx+6 etc.
+ *   variable slot v (0-based) starts at f + v*mx*my*mz.
+ * Only the mvar evolved variables are transferred; slots are given by
+ * pcmx_params (registration order, src/register.f90:137-146).
+ *
+ * Threading: all calls from one host thread (the OpenMP master, as in the
+ * reference); device work is ordered on one library-owned HIP stream.
+ */
+#ifndef PCMX_H
+#define PCMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCMX_OK 0
+#define PCMX_ERR_ARG (-1)      /* invalid argument or unsupported configuration */
+#define PCMX_ERR_STATE (-2)    /* called before pcmx_initialize / after finalize */
+#define PCMX_ERR_HIP (-3)      /* HIP runtime error (message names the call) */
+#define PCMX_ERR_NOTFOUND (-4) /* update by name: unknown parameter */
+
+#define PCMX_PREC_FP64 0  /* stencils, RHS and state in double */
+#define PCMX_PREC_MIXED 1 /* stencils + RHS in float on offsets from the box
+                             mean; state, RK accumulation, dt and diagnostics
+                             in double */
+
+#define PCMX_N_REDUCED 20 /* n_gpu_reduced_vars, src/gpu.h:12 */
+
+/* Run parameters.  Replaces the generated-header parameter export of the
+ * reference (pushpars2c / copy_addr, gpu_astaroth.f90:503-518) with an
+ * explicit argument list. */
+typedef struct pcmx_params {
+  int nx, ny, nz;            /* interior grid (cparam.local nxgrid...; one process) */
+  int mvar;                  /* evolved variables: 4 hydro, 5 +entropy, 7 MHD, 8 MHD+entropy */
+  int iux, ilnrho, iss, iax; /* 0-based slots; iss / iax = -1 when absent */
+  double dx, dy, dz;         /* grid spacing (grid.f90:123-125) */
+  double nu;                 /* viscosity_run_pars nu, ivisc='nu-const' */
+  double eta;                /* magnetic_run_pars eta, iresistivity='eta-const' */
+  double chi;                /* entropy_run_pars chi, iheatcond='chi-const' */
+  double cs0, gamma, cp, rho0, mu0; /* eos_init_pars / cdata */
+  double cdt, cdtv;          /* run_pars Courant factors (cdata.f90:235-239) */
+  int ldt;                   /* 1: Courant-limited dt (run_pars dt=0, timestep.f90:67) */
+  int precision;             /* PCMX_PREC_* */
+  int kernel;                /* 0: auto, 1: direct (one thread per cell), 2: z-marching LDS */
+} pcmx_params;
+
+/* Last error message (thread-local to the calling thread). */
+const char* pcmx_last_error(void);
+
+/* register_GPU -> register_gpu_c (gpu_astaroth_ansi.c:125-130).  No-op. */
+int pcmx_register(void);
+
+/* initialize_GPU -> initialize_gpu_c(f, comm, t, nt, ...)
+ * (gpu_astaroth.f90:192-245, gpu_astaroth_ansi.c:110-124, gpu_astaroth.cc:2009-2122).
+ * Allocates device state for the grid and physics in *par.  When nt > 0 the
+ * host f (layout above, dims mx,my,mz,mfarray) is loaded, as initialize_GPU
+ * does through load_farray_to_GPU.  f may be NULL when nt == 0. */
+int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double t, int nt);
+
+/* finalize_GPU -> finalize_gpu_c (gpu_astaroth_ansi.c:131-136, gpu_astaroth.cc:2243-2272). */
+int pcmx_finalize(void);
+
+/* load_farray_to_GPU -> load_farray_c (gpu_astaroth.f90:414-421, gpu_astaroth.cc:1947-1971).
+ * Copies the mvar evolved variables of the host f into device state; ghost
+ * zones of f are ignored (the device keeps the periodic interior only). */
+int pcmx_load_farray(const double* f, int mfarray);
+
+/* copy_farray_from_GPU -> copy_farray_c (gpu_astaroth.f90:390-412, gpu_astaroth.cc:1719-1814).
+ * Writes the device state into the host f, interior and periodic ghost zones
+ * of the mvar evolved variables. */
+int pcmx_copy_farray(double* f, int mfarray);
+
+/* before_boundary_gpu -> before_boundary_gpu_c (gpu_astaroth.f90:297-313,
+ * gpu_astaroth.cc:1320-1353).  Periodic ghosts are formed inside the RHS
+ * kernel, so nothing is done; kept for the call sequence of equ.f90:198. */
+int pcmx_before_boundary(int lrmv, int isubstep, double t);
+
+/* rhs_GPU -> rhs_gpu_c(isubstep, t) -> substepGPU (gpu_astaroth.f90:285-295,
+ * gpu_astaroth_ansi.c:170-208, gpu_astaroth.cc:1672-1717).
+ * Runs RK substep isubstep (1..3): RHS of every evolved variable, the 2N
+ * accumulation df = alpha*df + RHS and the update f += dt*beta*df
+ * (timestep.f90:110-161).  *dt is the step; when par->ldt is set and
+ * isubstep == 1, the library computes dt = 1/max(dt1) from the state before
+ * the update (equ.f90:408, sub.f90:4223-4254) and returns it in *dt.
+ * On isubstep 1 the urms/umax/brms diagnostics of the pre-update state are
+ * reduced on the device (readable with pcmx_get_reduced_vars). */
+int pcmx_rhs(int isubstep, double t, double* dt);
+
+/* update_after_substep_gpu -> update_after_substep_gpu_c (gpu_astaroth.cc:1515-1533).  No-op. */
+int pcmx_update_after_substep(void);
+
+/* gpu_prepare_for_first_substep -> gpu_prepare_for_first_substep_c
+ * (gpu_astaroth.f90:321-325, gpu_astaroth.cc:2456-2482).  Computes dt1_max of
+ * the current state without advancing it; returned in *dt1 (may be NULL). */
+int pcmx_prepare_for_first_substep(double t, double* dt1);
+
+/* get_gpu_reduced_vars -> get_gpu_reduced_vars_c (gpu_astaroth.f90:481-487,
+ * gpu_astaroth.cc:2486-2530).  dst[PCMX_N_REDUCED] receives, from the last
+ * substep 1: 0 urms, 1 umax, 2 brms, 3 dt1_max, 4 dt, 5.. box means of the
+ * mvar variables; unused slots are zero. */
+int pcmx_get_reduced_vars(double* dst);
+
+/* update_on_gpu(index,varname,value) -> update_on_gpu_scal_by_name_c /
+ * _by_ind_c (gpu_astaroth.f90:450-473, gpu_astaroth_ansi.c:236-259).
+ * Names: nu, eta, chi, cs0, cdt, cdtv.  By name returns the index (>= 0) or
+ * PCMX_ERR_NOTFOUND, like updateInConfigScalName. */
+int pcmx_update_by_name(const char* varname, double value);
+int pcmx_update_by_index(int index, double value);
+
+/* test_gpu_bcs -> test_bcs_c (gpu_astaroth.f90:489-493, gpu_astaroth.cc:2283-2454).
+ * Fills a random state, forms the periodic ghost zones on the device and on
+ * the host and returns the largest difference in *maxdiff (0 when equal). */
+int pcmx_test_bcs(double* maxdiff);
+
+/* split_update_gpu -> split_update_gpu_c (gpu_astaroth.f90:495-501).  No-op
+ * (no implicit / split physics on this path). */
+int pcmx_split_update(void);
+
+/* Advance nsteps full RK3 steps on the device with no host synchronisation
+ * between substeps (the loop of timestep.f90:206-273 without the host
+ * between substeps); *t and *dt are updated as the reference advances them.
+ * With ldiag, urms/umax/brms are reduced on every step's first substep. */
+int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag);
+
+/* Launch statistics of the last pcmx_time_step: averages in milliseconds of
+ * the fused RHS kernel per substep, measured with HIP events on the library
+ * stream.  out[0] = mean kernel ms, out[1] = number of kernels timed. */
+int pcmx_kernel_timing(double* out);
+int pcmx_set_timing(int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

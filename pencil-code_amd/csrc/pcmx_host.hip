@@ -1,0 +1,600 @@
+// pcmx_host.hip -- host driver and C ABI of libpcmx (declared in include/pcmx.h).
+//
+// One engine per process, like the reference's `module GPU` singleton
+// (src/gpu_astaroth.f90).  Device state: two ping-pong copies of the evolved
+// variables (the fused kernel reads the stencil of f while writing the
+// update, so it cannot update in place) and the 2N register w, each stored
+// interior-only as [var][z][y][x] doubles; periodic ghosts are formed by the
+// kernels.  The 2N-RK3 loop of timestep.f90:77-204 runs here, one fused
+// kernel per substep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pcmx.h"
+#include "pcmx_direct.h"
+#include "pcmx_kernels.h"
+#include "pcmx_zmarch.h"
+
+using namespace pcmx;
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHECK(call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail(PCMX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// Williamson (1980) 2N-RK3, timestep.f90:43-44.
+const double ALPHA_TS[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+const double BETA_TS[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+
+struct Engine {
+  pcmx_params par{};
+  Phys phys{};
+  long nxyz = 0;
+  int nvar = 0;
+  bool mag = false, ent = false;
+  double* f[2] = {nullptr, nullptr};
+  double* w = nullptr;
+  int cur = 0;
+  double* partials = nullptr;
+  long partials_cap = 0;
+  double* results = nullptr;  // R_COUNT doubles
+  double* staging = nullptr;  // one variable with ghosts (mw doubles)
+  hipStream_t stream = nullptr;
+  int kernel = 1;             // resolved kernel variant
+  ZPlan zplan{};
+  // timing of the fused kernel (pcmx_time_step)
+  bool timing = false;
+  double timing_ms = 0.0;
+  long timing_n = 0;
+  double last_dt = 0.0;
+};
+
+Engine* g = nullptr;
+
+int check_params(const pcmx_params* p) {
+  if (!p) return fail(PCMX_ERR_ARG, "params is NULL");
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(PCMX_ERR_ARG, "grid %dx%dx%d", p->nx, p->ny, p->nz);
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3)
+    return fail(PCMX_ERR_ARG, "periodic 6th-order stencils need >= 3 interior points per direction");
+  const bool ent = p->iss >= 0, mag = p->iax >= 0;
+  const int expect = 4 + (ent ? 1 : 0) + (mag ? 3 : 0);
+  if (p->mvar != expect) return fail(PCMX_ERR_ARG, "mvar=%d but slots imply %d", p->mvar, expect);
+  // registration order (register.f90:137-146): uu, lnrho, [ss], [aa]
+  if (p->iux != 0 || p->ilnrho != 3 || (ent && p->iss != 4) || (mag && p->iax != (ent ? 5 : 4)))
+    return fail(PCMX_ERR_ARG, "variable slots must follow the registration order uu,lnrho,[ss],[aa]");
+  if (!(p->dx > 0 && p->dy > 0 && p->dz > 0)) return fail(PCMX_ERR_ARG, "grid spacing must be positive");
+  if (p->precision != PCMX_PREC_FP64 && p->precision != PCMX_PREC_MIXED)
+    return fail(PCMX_ERR_ARG, "unknown precision %d", p->precision);
+  if (ent && p->gamma == 1.0) return fail(PCMX_ERR_ARG, "entropy needs gamma /= 1");
+  if (p->cdt <= 0 || p->cdtv <= 0) return fail(PCMX_ERR_ARG, "cdt and cdtv must be positive");
+  if (p->kernel < 0 || p->kernel > 2) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
+  return PCMX_OK;
+}
+
+// Derived constants, computed in the reference's operation order
+// (grid.f90:1045-1068, deriv.f90, eos_idealgas.f90:159-268, magnetic.f90:1553).
+Phys make_phys(const pcmx_params& p) {
+  Phys P{};
+  P.dx_1 = 1.0 / p.dx;
+  P.dy_1 = 1.0 / p.dy;
+  P.dz_1 = 1.0 / p.dz;
+  P.dxyz_2 = P.dx_1 * P.dx_1 + P.dy_1 * P.dy_1 + P.dz_1 * P.dz_1;
+  const double a = 1.0 / 60.0;
+  P.fac1[0] = a * P.dx_1;
+  P.fac1[1] = a * P.dy_1;
+  P.fac1[2] = a * P.dz_1;
+  P.fac2[0] = P.dx_1 * P.dx_1;
+  P.fac2[1] = P.dy_1 * P.dy_1;
+  P.fac2[2] = P.dz_1 * P.dz_1;
+  P.facxy = (1.0 / 720.0) * P.dx_1 * P.dy_1;
+  P.facxz = (1.0 / 720.0) * P.dz_1 * P.dx_1;
+  P.facyz = (1.0 / 720.0) * P.dy_1 * P.dz_1;
+  P.nu = p.nu;
+  P.eta = p.eta;
+  P.chi = p.chi;
+  P.cp = p.cp;
+  P.mu0 = p.mu0;
+  P.mu01 = 1. / p.mu0;
+  P.gamma = p.gamma;
+  P.gamma_m1 = p.gamma - 1.0;
+  P.gamma1 = 1 / p.gamma;
+  P.lnrho0 = std::log(p.rho0);
+  P.cs20 = p.cs0 * p.cs0;
+  const double cp1 = 1 / p.cp;
+  P.cv1 = p.gamma * cp1;
+  P.lnTT0 = (P.gamma_m1 != 0.0) ? std::log(P.cs20 / (p.cp * P.gamma_m1)) : std::log(P.cs20 / p.cp);
+  P.cdt = p.cdt;
+  P.cdtv = p.cdtv;
+  P.polytropic = (p.iss < 0 && p.gamma != 1.0) ? 1 : 0;
+  return P;
+}
+
+// ---------------------------------------------------------------- launches
+template <typename T, bool MAG, bool ENT, bool DT>
+void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
+  dim3 blk(64, 4, 1), grd((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
+  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT>), grd, blk, 0, s, a, P);
+}
+long direct_blocks(const pcmx_params& p) { return (long)((p.nx + 63) / 64) * ((p.ny + 3) / 4) * p.nz; }
+
+template <typename T, bool MAG, bool ENT>
+void launch_t(const Engine& e, const KArgs& a, bool dt) {
+  if (e.kernel == 2) {
+    if (dt) launch_zmarch<T, MAG, ENT, true>(a, e.phys, e.zplan, e.stream);
+    else launch_zmarch<T, MAG, ENT, false>(a, e.phys, e.zplan, e.stream);
+  } else {
+    if (dt) launch_direct<T, MAG, ENT, true>(a, e.phys, e.stream);
+    else launch_direct<T, MAG, ENT, false>(a, e.phys, e.stream);
+  }
+}
+
+template <typename T>
+void launch_p(const Engine& e, const KArgs& a, bool dt) {
+  if (e.mag) {
+    if (e.ent) launch_t<T, true, true>(e, a, dt);
+    else launch_t<T, true, false>(e, a, dt);
+  } else {
+    if (e.ent) launch_t<T, false, true>(e, a, dt);
+    else launch_t<T, false, false>(e, a, dt);
+  }
+}
+
+long nblocks(const Engine& e) { return e.kernel == 2 ? e.zplan.nblocks : direct_blocks(e.par); }
+
+int launch_rhs(Engine& e, const KArgs& a, bool dt) {
+  if (e.par.precision == PCMX_PREC_MIXED) launch_p<float>(e, a, dt);
+  else launch_p<double>(e, a, dt);
+  HIPCHECK(hipGetLastError());
+  return PCMX_OK;
+}
+
+KArgs base_args(Engine& e) {
+  KArgs a{};
+  a.nx = e.par.nx;
+  a.ny = e.par.ny;
+  a.nz = e.par.nz;
+  a.mvar = e.nvar;
+  a.iss = e.par.iss;
+  a.iax = e.par.iax;
+  a.nxyz = e.nxyz;
+  a.w = e.w;
+  a.means = e.results + R_MEAN0;
+  return a;
+}
+
+int finalize_red(Engine& e, bool set_dt, bool set_means) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, e.stream, e.partials, nblocks(e), e.results,
+                     (double)e.nxyz, e.nvar, set_dt ? 1 : 0, set_means ? 1 : 0);
+  HIPCHECK(hipGetLastError());
+  return PCMX_OK;
+}
+
+// One RK substep s (0-based) on the device; no host synchronisation.
+// dt_host is the fixed step (ignored when ldt).  diag requests the
+// reductions of the first substep.
+int substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  KArgs a = base_args(e);
+  a.fin = e.f[e.cur];
+  a.fout = e.f[1 - e.cur];
+  a.first = (s == 0);
+  a.alpha = ALPHA_TS[s];
+  a.beta = BETA_TS[s];
+  a.dtb = dt_host * BETA_TS[s];  // dt_beta_ts = dt*beta_ts (timestep.f90:150,215)
+  a.write_w = (s < 2);
+  const bool ldt = e.par.ldt != 0;
+  if (ldt && s == 0) {
+    // dt follows from this substep's RHS (equ.f90:408) before f is updated:
+    // store w1, reduce dt1, then f1 = f0 + dt*beta1*w1.
+    a.mode = MODE_W;
+    a.partials = e.partials;
+    if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
+    int rc = launch_rhs(e, a, true);
+    if (rc) return rc;
+    if (ev1) HIPCHECK(hipEventRecord(ev1, e.stream));
+    if ((rc = finalize_red(e, true, true))) return rc;
+    const long n = e.nxyz * e.nvar;
+    hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(256), 0, e.stream, e.f[e.cur], e.f[1 - e.cur], e.w,
+                       e.results + R_DT, BETA_TS[0], n);
+    HIPCHECK(hipGetLastError());
+  } else {
+    a.mode = MODE_UPDATE;
+    a.d_dt = ldt ? e.results + R_DT : nullptr;
+    a.partials = (s == 0 && diag) ? e.partials : nullptr;
+    if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
+    int rc = launch_rhs(e, a, false);
+    if (rc) return rc;
+    if (ev1) HIPCHECK(hipEventRecord(ev1, e.stream));
+    if (s == 0 && diag)
+      if ((rc = finalize_red(e, false, true))) return rc;
+  }
+  e.cur = 1 - e.cur;
+  return PCMX_OK;
+}
+
+int require() { return g ? PCMX_OK : fail(PCMX_ERR_STATE, "pcmx_initialize has not been called"); }
+
+void free_engine(Engine* e) {
+  if (!e) return;
+  for (double* p : {e->f[0], e->f[1], e->w, e->partials, e->results, e->staging})
+    if (p) (void)hipFree(p);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+long mw_of(const pcmx_params& p) { return (long)(p.nx + 6) * (p.ny + 6) * (p.nz + 6); }
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* pcmx_last_error(void) { return g_err; }
+
+int pcmx_register(void) { return PCMX_OK; }
+
+int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double t, int nt) {
+  (void)t;
+  int rc = check_params(par);
+  if (rc) return rc;
+  if (g) {
+    free_engine(g);
+    g = nullptr;
+  }
+  Engine* e = new Engine();
+  e->par = *par;
+  e->phys = make_phys(*par);
+  e->nxyz = (long)par->nx * par->ny * par->nz;
+  e->nvar = par->mvar;
+  e->mag = par->iax >= 0;
+  e->ent = par->iss >= 0;
+  // kernel choice: the z-marching LDS kernel needs whole tiles in x and y
+  int kern = par->kernel;
+  if (kern == 0) kern = zmarch_supported(*par) ? 2 : 1;
+  if (kern == 2 && !zmarch_supported(*par)) {
+    delete e;
+    return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
+  }
+  e->kernel = kern;
+  if (kern == 2) e->zplan = zmarch_plan(*par);
+  const size_t bytes = sizeof(double) * e->nxyz * e->nvar;
+  auto alloc = [&](double** p, size_t n) -> int {
+    hipError_t err = hipMalloc((void**)p, n);
+    if (err != hipSuccess)
+      return fail(PCMX_ERR_HIP, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(err));
+    return PCMX_OK;
+  };
+  e->partials_cap = std::max(direct_blocks(*par), e->kernel == 2 ? e->zplan.nblocks : 0L);
+  if ((rc = alloc(&e->f[0], bytes)) || (rc = alloc(&e->f[1], bytes)) || (rc = alloc(&e->w, bytes)) ||
+      (rc = alloc(&e->partials, sizeof(double) * NRED * e->partials_cap)) ||
+      (rc = alloc(&e->results, sizeof(double) * R_COUNT))) {
+    free_engine(e);
+    return rc;
+  }
+  hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (err != hipSuccess) {
+    free_engine(e);
+    return fail(PCMX_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
+  }
+  (void)hipMemsetAsync(e->w, 0, bytes, e->stream);
+  (void)hipMemsetAsync(e->results, 0, sizeof(double) * R_COUNT, e->stream);
+  g = e;
+  if (nt > 0) {
+    if (!f) return fail(PCMX_ERR_ARG, "nt > 0 needs the host f-array");
+    if ((rc = pcmx_load_farray(f, mfarray))) return rc;
+  }
+  return PCMX_OK;
+}
+
+int pcmx_finalize(void) {
+  if (g) {
+    (void)hipStreamSynchronize(g->stream);
+    free_engine(g);
+    g = nullptr;
+  }
+  return PCMX_OK;
+}
+
+int pcmx_load_farray(const double* f, int mfarray) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (!f) return fail(PCMX_ERR_ARG, "f is NULL");
+  if (mfarray < e.nvar) return fail(PCMX_ERR_ARG, "mfarray=%d < mvar=%d", mfarray, e.nvar);
+  const long mw = mw_of(e.par);
+  if (!e.staging) HIPCHECK(hipMalloc((void**)&e.staging, sizeof(double) * mw));
+  for (int v = 0; v < e.nvar; ++v) {
+    HIPCHECK(hipMemcpyAsync(e.staging, f + (long)v * mw, sizeof(double) * mw, hipMemcpyHostToDevice, e.stream));
+    hipLaunchKernelGGL(k_unpack_interior, dim3(1024), dim3(256), 0, e.stream, e.staging,
+                       e.f[e.cur] + (long)v * e.nxyz, e.par.nx, e.par.ny, e.par.nz);
+    HIPCHECK(hipGetLastError());
+  }
+  // box means for the mixed form, diagnostics and dt1 of the loaded state
+  KArgs a = base_args(e);
+  a.fin = e.f[e.cur];
+  a.fout = nullptr;
+  a.first = 1;
+  a.mode = MODE_DT;
+  a.partials = e.partials;
+  // the mixed form needs means before its first use: compute them with the
+  // FP64 direct kernel's reduction path
+  const int saved_kernel = e.kernel;
+  const int saved_prec = e.par.precision;
+  e.kernel = 1;
+  e.par.precision = PCMX_PREC_FP64;
+  rc = launch_rhs(e, a, true);
+  e.kernel = saved_kernel;
+  e.par.precision = saved_prec;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, e.stream, e.partials, direct_blocks(e.par), e.results,
+                     (double)e.nxyz, e.nvar, 0, 1);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  return PCMX_OK;
+}
+
+int pcmx_copy_farray(double* f, int mfarray) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (!f) return fail(PCMX_ERR_ARG, "f is NULL");
+  if (mfarray < e.nvar) return fail(PCMX_ERR_ARG, "mfarray=%d < mvar=%d", mfarray, e.nvar);
+  const long mw = mw_of(e.par);
+  if (!e.staging) HIPCHECK(hipMalloc((void**)&e.staging, sizeof(double) * mw));
+  for (int v = 0; v < e.nvar; ++v) {
+    hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, e.f[e.cur] + (long)v * e.nxyz,
+                       e.staging, e.par.nx, e.par.ny, e.par.nz);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(f + (long)v * mw, e.staging, sizeof(double) * mw, hipMemcpyDeviceToHost, e.stream));
+    HIPCHECK(hipStreamSynchronize(e.stream));
+  }
+  return PCMX_OK;
+}
+
+int pcmx_before_boundary(int lrmv, int isubstep, double t) {
+  (void)lrmv;
+  (void)isubstep;
+  (void)t;
+  return require();
+}
+
+int pcmx_rhs(int isubstep, double t, double* dt) {
+  (void)t;
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (isubstep < 1 || isubstep > 3) return fail(PCMX_ERR_ARG, "isubstep=%d (itorder=3)", isubstep);
+  if (!dt) return fail(PCMX_ERR_ARG, "dt is NULL");
+  const int s = isubstep - 1;
+  if (!e.par.ldt) {
+    e.last_dt = *dt;
+    double r[1] = {*dt};
+    HIPCHECK(hipMemcpyAsync(e.results + R_DT, r, sizeof(double), hipMemcpyHostToDevice, e.stream));
+  }
+  if ((rc = substep(e, s, *dt, true))) return rc;
+  if (e.par.ldt && s == 0) {
+    double r;
+    HIPCHECK(hipMemcpyAsync(&r, e.results + R_DT, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    HIPCHECK(hipStreamSynchronize(e.stream));
+    *dt = r;
+    e.last_dt = r;
+  } else {
+    HIPCHECK(hipStreamSynchronize(e.stream));
+  }
+  return PCMX_OK;
+}
+
+int pcmx_update_after_substep(void) { return require(); }
+int pcmx_split_update(void) { return require(); }
+
+int pcmx_prepare_for_first_substep(double t, double* dt1) {
+  (void)t;
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  KArgs a = base_args(e);
+  a.fin = e.f[e.cur];
+  a.first = 1;
+  a.mode = MODE_DT;
+  a.partials = e.partials;
+  if ((rc = launch_rhs(e, a, true))) return rc;
+  if ((rc = finalize_red(e, false, false))) return rc;
+  double r[R_COUNT];
+  HIPCHECK(hipMemcpyAsync(r, e.results, sizeof(r), hipMemcpyDeviceToHost, e.stream));
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  if (dt1) *dt1 = r[R_DT1];
+  return PCMX_OK;
+}
+
+int pcmx_get_reduced_vars(double* dst) {
+  int rc = require();
+  if (rc) return rc;
+  if (!dst) return fail(PCMX_ERR_ARG, "dst is NULL");
+  Engine& e = *g;
+  double r[R_COUNT];
+  HIPCHECK(hipMemcpyAsync(r, e.results, sizeof(r), hipMemcpyDeviceToHost, e.stream));
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  for (int i = 0; i < PCMX_N_REDUCED; ++i) dst[i] = 0.0;
+  dst[0] = r[R_URMS];
+  dst[1] = r[R_UMAX];
+  dst[2] = r[R_BRMS];
+  dst[3] = r[R_DT1];
+  dst[4] = e.par.ldt ? r[R_DT] : e.last_dt;
+  for (int v = 0; v < e.nvar; ++v) dst[5 + v] = r[R_MEAN0 + v];
+  return PCMX_OK;
+}
+
+static const char* const kParNames[] = {"nu", "eta", "chi", "cs0", "cdt", "cdtv"};
+
+int pcmx_update_by_index(int index, double value) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  switch (index) {
+    case 0: e.par.nu = value; break;
+    case 1: e.par.eta = value; break;
+    case 2: e.par.chi = value; break;
+    case 3: e.par.cs0 = value; break;
+    case 4: e.par.cdt = value; break;
+    case 5: e.par.cdtv = value; break;
+    default: return fail(PCMX_ERR_NOTFOUND, "no parameter with index %d", index);
+  }
+  if ((rc = check_params(&e.par))) return rc;
+  e.phys = make_phys(e.par);
+  return PCMX_OK;
+}
+
+int pcmx_update_by_name(const char* varname, double value) {
+  if (!varname) return fail(PCMX_ERR_ARG, "varname is NULL");
+  for (int i = 0; i < (int)(sizeof(kParNames) / sizeof(kParNames[0])); ++i)
+    if (std::strcmp(varname, kParNames[i]) == 0) {
+      int rc = pcmx_update_by_index(i, value);
+      return rc ? rc : i;
+    }
+  return fail(PCMX_ERR_NOTFOUND, "variable \"%s\" not found", varname);
+}
+
+int pcmx_test_bcs(double* maxdiff) {
+  int rc = require();
+  if (rc) return rc;
+  if (!maxdiff) return fail(PCMX_ERR_ARG, "maxdiff is NULL");
+  Engine& e = *g;
+  const pcmx_params& p = e.par;
+  const long n = e.nxyz, mw = mw_of(p);
+  std::vector<double> in(n), out(mw);
+  std::mt19937_64 rng(12345);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  for (auto& v : in) v = U(rng);
+  double* d_in = nullptr;
+  double* d_out = nullptr;
+  HIPCHECK(hipMalloc((void**)&d_in, sizeof(double) * n));
+  if (hipMalloc((void**)&d_out, sizeof(double) * mw) != hipSuccess) {
+    (void)hipFree(d_in);
+    return fail(PCMX_ERR_HIP, "hipMalloc failed in pcmx_test_bcs");
+  }
+  (void)hipMemcpy(d_in, in.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, d_in, d_out, p.nx, p.ny, p.nz);
+  (void)hipStreamSynchronize(e.stream);
+  (void)hipMemcpy(out.data(), d_out, sizeof(double) * mw, hipMemcpyDeviceToHost);
+  (void)hipFree(d_in);
+  (void)hipFree(d_out);
+  // host reference: boundconds_x, then y, then z over full extents
+  const int mx = p.nx + 6, my = p.ny + 6, mz = p.nz + 6;
+  std::vector<double> h(mw, 0.0);
+  for (int z = 0; z < p.nz; ++z)
+    for (int y = 0; y < p.ny; ++y)
+      for (int x = 0; x < p.nx; ++x)
+        h[((long)(z + 3) * my + y + 3) * mx + x + 3] = in[((long)z * p.ny + y) * p.nx + x];
+  for (int iz = 0; iz < mz; ++iz)
+    for (int iy = 0; iy < my; ++iy)
+      for (int gg = 0; gg < 3; ++gg) {
+        double* row = &h[((long)iz * my + iy) * mx];
+        row[gg] = row[p.nx + gg];
+        row[3 + p.nx + gg] = row[3 + gg];
+      }
+  for (int iz = 0; iz < mz; ++iz)
+    for (int gg = 0; gg < 3; ++gg)
+      for (int ix = 0; ix < mx; ++ix) {
+        h[((long)iz * my + gg) * mx + ix] = h[((long)iz * my + p.ny + gg) * mx + ix];
+        h[((long)iz * my + 3 + p.ny + gg) * mx + ix] = h[((long)iz * my + 3 + gg) * mx + ix];
+      }
+  for (int gg = 0; gg < 3; ++gg)
+    for (long q = 0; q < (long)mx * my; ++q) {
+      h[(long)gg * mx * my + q] = h[(long)(p.nz + gg) * mx * my + q];
+      h[(long)(3 + p.nz + gg) * mx * my + q] = h[(long)(3 + gg) * mx * my + q];
+    }
+  double md = 0.0;
+  for (long q = 0; q < mw; ++q) md = std::max(md, std::fabs(h[q] - out[q]));
+  *maxdiff = md;
+  return PCMX_OK;
+}
+
+int pcmx_set_timing(int enable) {
+  int rc = require();
+  if (rc) return rc;
+  g->timing = enable != 0;
+  return PCMX_OK;
+}
+
+int pcmx_kernel_timing(double* out) {
+  int rc = require();
+  if (rc) return rc;
+  if (!out) return fail(PCMX_ERR_ARG, "out is NULL");
+  out[0] = g->timing_n ? g->timing_ms / g->timing_n : 0.0;
+  out[1] = (double)g->timing_n;
+  return PCMX_OK;
+}
+
+int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (nsteps < 0 || !t || !dt) return fail(PCMX_ERR_ARG, "bad arguments to pcmx_time_step");
+  const bool ldt = e.par.ldt != 0;
+  if (!ldt) {
+    double r[1] = {*dt};
+    HIPCHECK(hipMemcpyAsync(e.results + R_DT, r, sizeof(double), hipMemcpyHostToDevice, e.stream));
+    e.last_dt = *dt;
+  }
+  std::vector<hipEvent_t> evs;
+  if (e.timing) {
+    evs.resize(6L * nsteps);
+    for (auto& ev : evs) HIPCHECK(hipEventCreate(&ev));
+  }
+  for (int it = 0; it < nsteps; ++it) {
+    double ds = 0.0;
+    for (int s = 0; s < 3; ++s) {
+      hipEvent_t e0 = e.timing ? evs[6L * it + 2 * s] : nullptr;
+      hipEvent_t e1 = e.timing ? evs[6L * it + 2 * s + 1] : nullptr;
+      if ((rc = substep(e, s, *dt, ldiag != 0, e0, e1))) return rc;
+      if (ldt && s == 0) {
+        // t advances with the step of this substep's Courant condition
+        double r;
+        HIPCHECK(hipMemcpyAsync(&r, e.results + R_DT, sizeof(double), hipMemcpyDeviceToHost, e.stream));
+        HIPCHECK(hipStreamSynchronize(e.stream));
+        *dt = r;
+        e.last_dt = r;
+      }
+      ds = (s == 0) ? 1.0 : ALPHA_TS[s] * ds + 1.0;  // timestep.f90:108-149
+      *t = *t + ds * (*dt * BETA_TS[s]);
+    }
+  }
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  if (e.timing) {
+    double tot = 0.0;
+    for (int it = 0; it < nsteps; ++it)
+      for (int s = 0; s < 3; ++s) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, evs[6L * it + 2 * s], evs[6L * it + 2 * s + 1]));
+        tot += ms;
+      }
+    e.timing_ms = tot;
+    e.timing_n = 3L * nsteps;
+    for (auto& ev : evs) (void)hipEventDestroy(ev);
+  }
+  return PCMX_OK;
+}
+
+}  // extern "C"

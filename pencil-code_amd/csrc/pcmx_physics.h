@@ -1,0 +1,237 @@
+// pcmx_physics.h -- pointwise RHS algebra of the fused kernels.
+//
+// Given the value, first, second and (bidiagonal) mixed derivatives of every
+// evolved variable at one cell, adds the right-hand side of
+//   src/hydro.f90 (duu_dt: -u.grad u), src/viscosity.f90 (nu-const force,
+//   viscous heat), src/density.f90 (continuity for ln rho),
+//   src/noentropy.f90 / src/entropy.f90 (pressure force, -u.grad s,
+//   chi-const conduction) with src/eos_idealgas.f90, and src/magnetic.f90
+//   (eta-const induction, J x B / rho, ohmic heat)
+// to the 2N register w, in the order calc_all_rhs adds them
+// (src/equ.f90:1268-1274: duu_dt, dlnrho_dt, denergy_dt, daa_dt), and
+// evaluates the inverse Courant step of set_dt1_max (src/equ.f90:1858-1912).
+// The operation grouping follows the reference expressions cited below so
+// that the FP64 form agrees with the reference to rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pcmx {
+
+// Constants of one box (host-computed once; identical for all kernels).
+struct Phys {
+  double dx_1, dy_1, dz_1, dxyz_2;
+  // stencil prefactors, computed in the reference's operation order:
+  double fac1[3];   // (1/60)*d?_1            deriv.f90:103,124,135,153
+  double fac2[3];   // d?_1**2                deriv.f90:498,514,539
+  double facxy;     // (1./720.)*dx_1*dy_1    deriv.f90:1409
+  double facxz;     // (1./720.)*dz_1*dx_1    deriv.f90:1439
+  double facyz;     // (1./720.)*dy_1*dz_1    deriv.f90:1424
+  double nu, eta, chi, cp, mu0, mu01;
+  double cs20, gamma, gamma_m1, gamma1, cv1, lnrho0, lnTT0;
+  double cdt, cdtv;
+  int polytropic;   // noentropy with gamma /= 1 (ilnrho_ss, leos_isentropic)
+};
+
+// der2_coef*, deriv.f90:36-40 ('standard').
+#define PCMX_D2C0 (-490.0 / 180.0)
+#define PCMX_D2C1 (270.0 / 180.0)
+#define PCMX_D2C2 (-27.0 / 180.0)
+#define PCMX_D2C3 (2.0 / 180.0)
+
+// Derivatives at one cell.  mix?[pair][c]: pair 0 = xy, 1 = xz, 2 = yz,
+// component c of the vector; only the entries the RHS uses are filled.
+template <typename T>
+struct Derivs {
+  T u[3], lnrho, ss, a[3];
+  T uij[3][3];        // uij[i][j] = D_j u_i           (gij, sub.f90:1505)
+  T glnrho[3], gss[3];
+  T aij[3][3];        // D_j A_i
+  T d2u[3][3];        // D2_j u_i                      (del2v_etc fijj)
+  T d2lnrho[3], d2ss[3];
+  T d2a[3][3];        // D2_j A_i                      (gij_etc d2A(j,j,i))
+  T mxu[3][3];        // mixed of u: [0][0..1] xy of u_x,u_y; [1][0],[1][2] xz; [2][1],[2][2] yz
+  T mxa[3][3];        // same for A
+};
+
+template <typename T>
+__device__ __forceinline__ T pexp(T x);
+template <>
+__device__ __forceinline__ double pexp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float pexp<float>(float x) { return __expf(x); }
+
+template <typename T>
+__device__ __forceinline__ T psqrt(T x);
+template <>
+__device__ __forceinline__ double psqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float psqrt<float>(float x) { return sqrtf(x); }
+
+// multm2_sym_mn, sub.f90:1207-1226: b = s11^2; then per row i>1:
+// b += s_ii^2; b += 2*s_ij^2 (j<i).
+template <typename T>
+__device__ __forceinline__ T d_sij2(const T (&s)[3][3]) {
+  T b = s[0][0] * s[0][0];
+  b = b + s[1][1] * s[1][1];
+  b = b + (T)2 * (s[1][0] * s[1][0]);
+  b = b + s[2][2] * s[2][2];
+  b = b + (T)2 * (s[2][0] * s[2][0]);
+  b = b + (T)2 * (s[2][1] * s[2][1]);
+  return b;
+}
+
+// Adds the RHS to w[] (already alpha-scaled) and returns the cell's
+// dt1_max_loc in *dt1 (only meaningful when DT), u2 and b2 for diagnostics.
+// NV slots: 0..2 uu, 3 lnrho, [4 ss], then aa.
+template <typename T, bool MAG, bool ENT, bool DT>
+__device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, double* w, int iss,
+                                          int iax, double* dt1, double* u2_out, double* b2_out) {
+  const T dx_1 = (T)P.dx_1, dy_1 = (T)P.dy_1, dz_1 = (T)P.dz_1;
+  // ---- hydro pencils (hydro.f90:3925-4000; sub.f90 helpers) ----
+  const T u0 = d.u[0], u1 = d.u[1], u2v = d.u[2];
+  const T u2 = u0 * u0 + u1 * u1 + u2v * u2v;                           // dot2_mn
+  const T divu = d.uij[0][0] + d.uij[1][1] + d.uij[2][2];               // div_mn
+  T s[3][3];                                                            // traceless_strain
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    s[j][j] = d.uij[j][j] - (T)(1.0 / 3.0) * divu;
+#pragma unroll
+    for (int i = j + 1; i < 3; ++i) {
+      const T v = (T)0.5 * (d.uij[i][j] + d.uij[j][i]);
+      s[i][j] = v;
+      s[j][i] = v;
+    }
+  }
+  T ugu[3];                                                             // u_dot_grad_vec
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ugu[j] = u0 * d.uij[j][0] + u1 * d.uij[j][1] + u2v * d.uij[j][2];
+  T del2u[3], gdivu[3];                                                 // del2v_etc
+#pragma unroll
+  for (int i = 0; i < 3; ++i) del2u[i] = d.d2u[i][0] + d.d2u[i][1] + d.d2u[i][2];
+  gdivu[0] = d.d2u[0][0] + d.mxu[0][1] + d.mxu[1][2];
+  gdivu[1] = d.mxu[0][0] + d.d2u[1][1] + d.mxu[2][2];
+  gdivu[2] = d.mxu[1][0] + d.mxu[2][1] + d.d2u[2][2];
+
+  // ---- density pencils (density.f90:2727-2811) ----
+  const T lnrho = d.lnrho;
+  const T rho1 = pexp<T>(-lnrho);
+  const T uglnrho = u0 * d.glnrho[0] + u1 * d.glnrho[1] + u2v * d.glnrho[2];
+  T sglnrho[3];                                                         // multmv_mn
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    sglnrho[i] = s[i][0] * d.glnrho[0] + s[i][1] * d.glnrho[1] + s[i][2] * d.glnrho[2];
+
+  // ---- EOS + energy pencils ----
+  T cs2, TT1 = 0, glnTT[3] = {0, 0, 0}, del2lnTT = 0, ugss = 0;
+  T fpres[3];
+  if constexpr (!ENT) {
+    if (P.polytropic) cs2 = (T)P.cs20 * pexp<T>((T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
+    else cs2 = (T)P.cs20;                                               // eos_idealgas.f90:969-1000
+#pragma unroll
+    for (int j = 0; j < 3; ++j) fpres[j] = -cs2 * d.glnrho[j];          // noentropy.f90:252-300
+  } else {
+    const T ss = d.ss;
+    const T del2lnrho = d.d2lnrho[0] + d.d2lnrho[1] + d.d2lnrho[2];
+    const T del2ss = d.d2ss[0] + d.d2ss[1] + d.d2ss[2];
+    const T dl = lnrho - (T)P.lnrho0;
+    cs2 = (T)P.cs20 * pexp<T>((T)P.cv1 * ss + (T)P.gamma_m1 * dl);       // eos_idealgas.f90:871
+    const T lnTT = (T)P.lnTT0 + (T)P.cv1 * ss + (T)P.gamma_m1 * dl;      // :876
+    TT1 = pexp<T>(-lnTT);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) glnTT[j] = (T)P.gamma_m1 * d.glnrho[j] + (T)P.cv1 * d.gss[j];
+    del2lnTT = (T)P.gamma_m1 * del2lnrho + (T)P.cv1 * del2ss;
+    ugss = u0 * d.gss[0] + u1 * d.gss[1] + u2v * d.gss[2];               // entropy.f90:3563
+#pragma unroll
+    for (int j = 0; j < 3; ++j)                                         // entropy.f90:3580
+      fpres[j] = -(cs2 * (d.glnrho[j] + glnTT[j]) * (T)P.gamma1);
+  }
+
+  // ---- viscosity, nu-const (viscosity.f90:1633-1663) ----
+  T fvisc[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    fvisc[j] = (T)P.nu * (del2u[j] + (T)2. * sglnrho[j] + (T)(1. / 3.) * gdivu[j]);
+
+  // ---- magnetic pencils (magnetic.f90:4279-5225; sub.f90:2945-3113) ----
+  T bb[3] = {0, 0, 0}, jxbr[3] = {0, 0, 0}, uxb[3] = {0, 0, 0}, del2a[3] = {0, 0, 0};
+  T b2 = 0, j2 = 0, advec_va2 = 0;
+  if constexpr (MAG) {
+    bb[0] = d.aij[2][1] - d.aij[1][2];                                  // curl_mn
+    bb[1] = d.aij[0][2] - d.aij[2][0];
+    bb[2] = d.aij[1][0] - d.aij[0][1];
+    b2 = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
+    uxb[0] = u1 * bb[2] - u2v * bb[1];                                  // cross_mn
+    uxb[1] = u2v * bb[0] - u0 * bb[2];
+    uxb[2] = u0 * bb[1] - u1 * bb[0];
+    // curlb = curl_mn(bij) with bij(i,j) = eps_ikl d2A(k,j,l)
+    T curlb[3];
+    curlb[0] = (d.mxa[0][1] - d.d2a[0][1]) - (d.d2a[0][2] - d.mxa[1][2]);
+    curlb[1] = (d.mxa[2][2] - d.d2a[1][2]) - (d.d2a[1][0] - d.mxa[0][0]);
+    curlb[2] = (d.mxa[1][0] - d.d2a[2][0]) - (d.d2a[2][1] - d.mxa[2][1]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) del2a[i] = d.d2a[i][0] + d.d2a[i][1] + d.d2a[i][2];
+    T jj[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) jj[i] = (T)P.mu01 * curlb[i];
+    if constexpr (ENT) j2 = jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2];
+    T jxb[3];
+    jxb[0] = jj[1] * bb[2] - jj[2] * bb[1];
+    jxb[1] = jj[2] * bb[0] - jj[0] * bb[2];
+    jxb[2] = jj[0] * bb[1] - jj[1] * bb[0];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) jxbr[i] = rho1 * jxb[i];
+    if constexpr (DT) {
+      const T b1x = bb[0] * dx_1, b1y = bb[1] * dy_1, b1z = bb[2] * dz_1;
+      advec_va2 = (b1x * b1x + b1y * b1y + b1z * b1z) * (T)P.mu01 * rho1;  // :5180
+    }
+  }
+
+  // ---- calc_all_rhs: duu_dt, dlnrho_dt, denergy_dt, daa_dt ----
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double acc = w[j];
+    acc = acc - (double)ugu[j];        // hydro.f90:4384
+    acc = acc + (double)fvisc[j];      // viscosity.f90:2823
+    acc = acc + (double)fpres[j];      // noentropy.f90:381 / entropy.f90:3714
+    if constexpr (MAG) acc = acc + (double)jxbr[j];  // magnetic.f90:5531
+    w[j] = acc;
+  }
+  {
+    const T density_rhs = -divu - uglnrho;  // density.f90:2929-2930
+    w[3] = w[3] + (double)density_rhs;
+  }
+  if constexpr (ENT) {
+    const T visc_heat = (T)2 * (T)P.nu * d_sij2(s);
+    double acc = w[iss];
+    acc = acc - (double)ugss;                               // entropy.f90:3758
+    acc = acc + (double)(TT1 * visc_heat);                  // viscosity.f90:2764
+    const T a0 = d.glnrho[0] + glnTT[0], a1 = d.glnrho[1] + glnTT[1], a2 = d.glnrho[2] + glnTT[2];
+    const T g2 = a0 * glnTT[0] + a1 * glnTT[1] + a2 * glnTT[2];
+    acc = acc + (double)((T)P.chi * (T)P.cp * (del2lnTT + g2));  // entropy.f90:4852-4947
+    if constexpr (MAG) acc = acc + (double)((T)P.eta * (T)P.mu0 * j2 * rho1 * TT1);  // magnetic.f90:6143
+    w[iss] = acc;
+  }
+  if constexpr (MAG) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) w[iax + j] = w[iax + j] + (double)(uxb[j] + (T)P.eta * del2a[j]);  // :6247,6524
+  }
+
+  // ---- set_dt1_max (equ.f90:1858-1912) ----
+  if constexpr (DT) {
+    const T advec_uu = fabs(u0) * dx_1 + fabs(u1) * dy_1 + fabs(u2v) * dz_1;  // hydro.f90:3500
+    const T advec_cs2 = cs2 * (T)P.dxyz_2;
+    const T advec2 = advec_va2 + advec_cs2;
+    const T maxadvec = advec_uu + psqrt<T>(advec2);
+    T maxdiffus = (T)P.nu * (T)P.dxyz_2;                                 // viscosity.f90:2435
+    if constexpr (MAG) maxdiffus = fmax(maxdiffus, (T)P.eta * (T)P.dxyz_2);
+    if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
+    const T dt1_advec = maxadvec / (T)P.cdt;
+    const T dt1_diffus = maxdiffus / (T)P.cdtv;
+    *dt1 = (double)psqrt<T>(dt1_advec * dt1_advec + dt1_diffus * dt1_diffus);
+  }
+  *u2_out = (double)u2;
+  *b2_out = (double)b2;
+}
+
+}  // namespace pcmx

@@ -1,0 +1,404 @@
+! $Id$
+!
+! MODULE_DOC: GPU module backed by libpcmx, a hand-written HIP engine for the
+! MODULE_DOC: MI355X (gfx950): fused 6th-order RHS + 2N-RK3 substep kernels.
+!
+! CPARAM logical, parameter :: lgpu = .true.
+!
+!**************************************************************************
+!
+!  Drop-in replacement of gpu_astaroth.f90 (select with GPU = gpu_mi355x in
+!  Makefile.local).  It exports the public list of src/gpu.h and talks to
+!  the device only through the C ABI of include/pcmx.h, bound with
+!  iso_c_binding in module pcmx_iface.  Supported physics (checked at
+!  initialize_GPU where the flags are visible): hydro + log density +
+!  (noentropy | entropy) + optional magnetic, nu-const viscosity, eta-const
+!  resistivity, chi-const heat conduction, periodic Cartesian equidistant
+!  grid, one process, itorder=3.
+!
+module GPU
+!
+  use Cdata
+  use General, only: keep_compiler_quiet
+  use Messages
+  use iso_c_binding
+  use pcmx_iface
+!
+  implicit none
+!
+  include 'gpu.h'
+!
+  logical :: ltest_bcs=.false.    !PAR-DOC: compare device and host periodic ghost zones at start
+  logical :: ltest_rhs=.false.    !PAR-DOC: run the reference's test_rhs_gpu comparison (equ.f90)
+  integer :: it_test_rhs=1
+  integer :: gpu_precision=0      !PAR-DOC: 0 FP64, 1 mixed (FP32 stencils on box-mean offsets)
+  integer :: gpu_kernel=0         !PAR-DOC: 0 auto, 1 direct, 2 z-marching LDS kernel
+!
+  namelist /gpu_run_pars/ ltest_bcs, ltest_rhs, it_test_rhs, gpu_precision, gpu_kernel
+!
+  real, dimension(n_gpu_reduced_vars) :: reduced_last = 0.
+!
+contains
+!***********************************************************************
+    subroutine pcmx_check(rc, caller)
+!
+!  Turns a negative libpcmx return code into fatal_error (the reference's
+!  C++ side prints and exits instead, gpu_astaroth.cc:2074-2078).
+!
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: caller
+!
+      if (rc < 0) call fatal_error(caller, trim(pcmx_error_message()))
+!
+    endsubroutine pcmx_check
+!***********************************************************************
+    subroutine register_GPU
+!
+      call pcmx_check(pcmx_register(), 'register_GPU')
+!
+    endsubroutine register_GPU
+!***********************************************************************
+    subroutine initialize_GPU(f)
+!
+!  Checks that the run uses only what the engine implements, passes the grid
+!  and physics parameters explicitly (no generated headers), allocates the
+!  device state and loads f (gpu_astaroth.f90:192-245).
+!
+      use SharedVariables, only: get_shared_variable
+      use EquationOfState, only: rho0
+!
+      real, dimension(:,:,:,:), intent(in) :: f
+!
+      type(pcmx_params) :: par
+      real, pointer :: nu_p, eta_p, chi_p, cs20_p, gamma_p, cp_p
+      character(len=512) :: str
+!
+      str=''
+      if (ncpus /= 1) str=trim(str)//', ncpus>1'
+      if (.not. all(lperi)) str=trim(str)//', non-periodic boundaries'
+      if (.not. lcartesian_coords) str=trim(str)//', non-Cartesian coordinates'
+      if (.not. all(lequidist)) str=trim(str)//', non-equidistant grid'
+      if (.not. (lhydro .and. ldensity)) str=trim(str)//', needs hydro and density'
+      if (ldensity_nolog) str=trim(str)//', linear density'
+      if (ltemperature .or. lthermal_energy) str=trim(str)//', temperature/thermal_energy'
+      if (itorder /= 3) str=trim(str)//', itorder/=3'
+      if (ldt .and. (ddt > 0. .or. loutput_varn_at_exact_tsnap)) str=trim(str)//', dt growth limiter'
+      if (str /= '') call fatal_error('initialize_GPU', &
+          'gpu_mi355x does not implement: '//trim(str(3:)))
+!
+      call get_shared_variable('nu', nu_p)
+      call get_shared_variable('cs20', cs20_p)
+      call get_shared_variable('gamma', gamma_p)
+      call get_shared_variable('cp', cp_p)
+!
+      par%nx = nx; par%ny = ny; par%nz = nz; par%mvar = mvar
+      par%iux = iux - 1; par%ilnrho = ilnrho - 1
+      par%iss = -1; par%iax = -1
+      if (lentropy) par%iss = iss - 1
+      if (lmagnetic) par%iax = iax - 1
+      par%dx = dx; par%dy = dy; par%dz = dz
+      par%nu = nu_p
+      par%eta = 0.; par%chi = 0.
+      if (lmagnetic) then
+        call get_shared_variable('eta', eta_p)
+        par%eta = eta_p
+      endif
+      if (lentropy) then
+        call get_shared_variable('chi', chi_p)
+        par%chi = chi_p
+      endif
+      par%cs0 = sqrt(cs20_p); par%gamma = gamma_p; par%cp = cp_p
+      par%rho0 = rho0; par%mu0 = mu0
+      par%cdt = cdt; par%cdtv = cdtv
+      par%ldt = merge(1, 0, ldt)
+      par%precision = gpu_precision
+      par%kernel = gpu_kernel
+!
+      call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
+      if (nt > 0) call load_farray_to_GPU(f)
+      if (ltest_bcs) call test_gpu_bcs
+!
+    endsubroutine initialize_GPU
+!***********************************************************************
+    subroutine read_gpu_run_pars(iomsg)
+!
+      use File_io, only: parallel_unit
+!
+      character(len=*), intent(out) :: iomsg
+      integer :: iostat
+!
+      read(parallel_unit, NML=gpu_run_pars, IOSTAT=iostat, IOMSG=iomsg)
+      if (iostat==0) iomsg=""
+!
+    endsubroutine read_gpu_run_pars
+!***********************************************************************
+    subroutine write_gpu_run_pars(unit)
+!
+      integer, intent(in) :: unit
+!
+      write(unit, NML=gpu_run_pars)
+!
+    endsubroutine write_gpu_run_pars
+!***********************************************************************
+    subroutine finalize_GPU
+!
+      call pcmx_check(pcmx_finalize(), 'finalize_GPU')
+!
+    endsubroutine finalize_GPU
+!***********************************************************************
+    subroutine get_farray_ptr_gpu
+!
+!  The device state is interior-only and stays private to the library.
+!
+    endsubroutine get_farray_ptr_gpu
+!***********************************************************************
+    subroutine rhs_GPU(f, isubstep)
+!
+!  One RK substep on the device: RHS, df = alpha*df + RHS, f += dt*beta*df
+!  (gpu_astaroth.f90:285-295).  With a Courant-limited step the engine
+!  computes dt1 from this step's first-substep RHS, before the update, as the
+!  CPU path does (equ.f90:408); set_dt then forms dt = 1/dt1 exactly as the
+!  device did.
+!
+      use Sub, only: set_dt
+!
+      real, dimension (mx,my,mz,mfarray), intent(inout) :: f
+      integer, intent(in) :: isubstep
+!
+      real(c_double) :: dt_c
+      real(c_double), dimension(PCMX_N_REDUCED) :: red
+!
+      call keep_compiler_quiet(f)
+      dt_c = dt
+      call pcmx_check(pcmx_rhs(isubstep, real(t, c_double), dt_c), 'rhs_GPU')
+      if (isubstep == 1) then
+        call pcmx_check(pcmx_get_reduced_vars(red), 'rhs_GPU')
+        reduced_last = red
+        if (ldt) call set_dt(real(red(4)))
+      endif
+!
+    endsubroutine rhs_GPU
+!***********************************************************************
+    subroutine before_boundary_gpu(f, lrmv, isubstep, t)
+!
+      real, dimension (mx,my,mz,mfarray), intent(inout) :: f
+      logical :: lrmv
+      integer, intent(in) :: isubstep
+      real(KIND=rkind8), intent(in) :: t
+!
+      call keep_compiler_quiet(f)
+      call pcmx_check(pcmx_before_boundary(merge(1, 0, lrmv), isubstep, real(t, c_double)), &
+                      'before_boundary_gpu')
+!
+    endsubroutine before_boundary_gpu
+!***********************************************************************
+    subroutine update_after_substep_gpu
+!
+      call pcmx_check(pcmx_update_after_substep(), 'update_after_substep_gpu')
+!
+    endsubroutine update_after_substep_gpu
+!***********************************************************************
+    subroutine gpu_prepare_for_first_substep
+!
+!  Initial dt for Courant-limited runs (gpu_astaroth.cc:2456-2482).
+!
+      use Sub, only: set_dt
+!
+      real(c_double) :: dt1
+!
+      call pcmx_check(pcmx_prepare_for_first_substep(real(t, c_double), dt1), &
+                      'gpu_prepare_for_first_substep')
+      if (ldt) call set_dt(real(dt1))
+!
+    endsubroutine gpu_prepare_for_first_substep
+!***********************************************************************
+    function get_ptr_GPU(ind1, ind2, lout) result(pFarr)
+!
+      integer, optional :: ind1, ind2
+      logical, optional :: lout
+      real, dimension(:,:,:,:), pointer :: pFarr
+!
+      nullify(pFarr)
+      call fatal_error('get_ptr_GPU', 'device arrays are not exposed by gpu_mi355x')
+!
+    endfunction get_ptr_GPU
+!***********************************************************************
+    function get_ptr_GPU_training(ind1, ind2, lout) result(pFarr)
+!
+      integer :: ind1
+      integer, optional :: ind2
+      logical, optional :: lout
+      real, dimension(:,:,:,:,:), pointer :: pFarr
+!
+      nullify(pFarr)
+      call fatal_error('get_ptr_GPU_training', 'training is not available with gpu_mi355x')
+!
+    endfunction get_ptr_GPU_training
+!***********************************************************************
+    subroutine copy_farray_from_GPU(f, nowait_)
+!
+!  Device state -> host f, interior and periodic ghosts of the mvar evolved
+!  variables (gpu_astaroth.f90:390-412).  The copy is synchronous.
+!
+      real, dimension (mx,my,mz,mfarray), intent(out) :: f
+      logical, optional :: nowait_
+!
+      call pcmx_check(pcmx_copy_farray(f, mfarray), 'copy_farray_from_GPU')
+!
+    endsubroutine copy_farray_from_GPU
+!***********************************************************************
+    subroutine load_farray_to_GPU(f)
+!
+      real, dimension (mx,my,mz,mfarray), intent(in) :: f
+!
+      call pcmx_check(pcmx_load_farray(f, mfarray), 'load_farray_to_GPU')
+!
+    endsubroutine load_farray_to_GPU
+!***********************************************************************
+    subroutine reload_GPU_config
+!
+!  RELOAD: push the possibly changed run parameters (gpu_astaroth.f90:423-427).
+!
+      use SharedVariables, only: get_shared_variable
+!
+      real, pointer :: p
+      integer(c_int) :: rc
+!
+      call get_shared_variable('nu', p); rc = pcmx_update_by_name('nu'//c_null_char, real(p, c_double))
+      call pcmx_check(rc, 'reload_GPU_config')
+      if (lmagnetic) then
+        call get_shared_variable('eta', p); rc = pcmx_update_by_name('eta'//c_null_char, real(p, c_double))
+        call pcmx_check(rc, 'reload_GPU_config')
+      endif
+      if (lentropy) then
+        call get_shared_variable('chi', p); rc = pcmx_update_by_name('chi'//c_null_char, real(p, c_double))
+        call pcmx_check(rc, 'reload_GPU_config')
+      endif
+      rc = pcmx_update_by_name('cdt'//c_null_char, real(cdt, c_double)); call pcmx_check(rc, 'reload_GPU_config')
+      rc = pcmx_update_by_name('cdtv'//c_null_char, real(cdtv, c_double)); call pcmx_check(rc, 'reload_GPU_config')
+!
+    endsubroutine reload_GPU_config
+!***********************************************************************
+    subroutine update_on_gpu_vec(index, varname, value)
+!
+      integer, intent(inout) :: index
+      character(len=*), optional :: varname
+      real, dimension(3), optional :: value
+!
+      call fatal_error('update_on_gpu_vec', 'no vector parameters on this path: '//trim(varname))
+!
+    endsubroutine update_on_gpu_vec
+!***********************************************************************
+    subroutine update_on_gpu(index, varname, value)
+!
+!  index<0: look the parameter up by name and return its index; an unknown
+!  name is fatal, as in gpu_astaroth.f90:450-473.
+!
+      integer, intent(inout) :: index
+      character(len=*), optional :: varname
+      real, optional :: value
+!
+      integer(c_int) :: rc
+!
+      if (.not. present(value)) return
+      if (index >= 0) then
+        call pcmx_check(pcmx_update_by_index(int(index, c_int), real(value, c_double)), 'update_on_gpu')
+      else
+        rc = pcmx_update_by_name(trim(varname)//c_null_char, real(value, c_double))
+        if (rc < 0) call fatal_error('update_on_gpu', 'variable "'//trim(varname)//'" not found')
+        index = rc
+      endif
+!
+    endsubroutine update_on_gpu
+!***********************************************************************
+    subroutine radtransfer_gpu
+!
+    endsubroutine radtransfer_gpu
+!***********************************************************************
+    subroutine get_gpu_reduced_vars(dst)
+!
+!  dst(1:5) = urms, umax, brms, dt1_max, dt of the last first substep;
+!  dst(6:) = box means of the evolved variables.
+!
+      real, dimension(20) :: dst
+      real(c_double), dimension(PCMX_N_REDUCED) :: red
+!
+      call pcmx_check(pcmx_get_reduced_vars(red), 'get_gpu_reduced_vars')
+      dst = red
+!
+    endsubroutine get_gpu_reduced_vars
+!***********************************************************************
+    subroutine test_gpu_bcs
+!
+      real(c_double) :: maxdiff
+!
+      call pcmx_check(pcmx_test_bcs(maxdiff), 'test_gpu_bcs')
+      if (lroot) print*, 'test_gpu_bcs: max |device - host| over ghost zones =', maxdiff
+      if (maxdiff /= 0.) call fatal_error('test_gpu_bcs', 'device ghost zones differ from host')
+!
+    endsubroutine test_gpu_bcs
+!***********************************************************************
+    subroutine split_update_gpu(f)
+!
+      real, dimension (mx,my,mz,mfarray), intent(inout) :: f
+!
+      call keep_compiler_quiet(f)
+      call pcmx_check(pcmx_split_update(), 'split_update_gpu')
+!
+    endsubroutine split_update_gpu
+!***********************************************************************
+    subroutine pushpars2c(p_par)
+!
+!  Parameters reach libpcmx through pcmx_params, not generated headers.
+!
+      integer, parameter :: n_pars=20
+      integer(KIND=ikind8), dimension(n_pars) :: p_par
+!
+      call keep_compiler_quiet(p_par(1))
+!
+    endsubroutine pushpars2c
+!***********************************************************************
+    subroutine TF_train_gpu(model_name, loss, itsub, t)
+      real :: loss
+      integer :: itsub
+      real(KIND=rkind8), intent(IN) :: t
+      character(len=*), intent(in) :: model_name
+      call fatal_error('TF_train_gpu', 'training is not available with gpu_mi355x')
+    endsubroutine TF_train_gpu
+!***********************************************************************
+    subroutine TF_infer_gpu(model_name, flag)
+      integer :: flag
+      character(len=*), intent(in) :: model_name
+      call fatal_error('TF_infer_gpu', 'training is not available with gpu_mi355x')
+    endsubroutine TF_infer_gpu
+!***********************************************************************
+    subroutine TF_create_model(model_name, config_file_path, lmpicomm)
+      logical :: lmpicomm
+      character(len=*), intent(in) :: model_name, config_file_path
+      call fatal_error('TF_create_model', 'training is not available with gpu_mi355x')
+    endsubroutine TF_create_model
+!***********************************************************************
+    subroutine tau_snapshots()
+    endsubroutine tau_snapshots
+!***********************************************************************
+    subroutine TF_load_model(model_name, fname)
+      character(len=*), intent(in) :: model_name, fname
+      call fatal_error('TF_load_model', 'training is not available with gpu_mi355x')
+    endsubroutine TF_load_model
+!***********************************************************************
+    subroutine TF_load_model_checkpoint(model_name, checkpoint_dir)
+      character(len=*), intent(in) :: model_name, checkpoint_dir
+      call fatal_error('TF_load_model_checkpoint', 'training is not available with gpu_mi355x')
+    endsubroutine TF_load_model_checkpoint
+!***********************************************************************
+    subroutine TF_save_model(model_name, fname)
+      character(len=*), intent(in) :: model_name, fname
+      call fatal_error('TF_save_model', 'training is not available with gpu_mi355x')
+    endsubroutine TF_save_model
+!***********************************************************************
+    subroutine TF_save_checkpoint(model_name, checkpoint_dir)
+      character(len=*), intent(in) :: model_name, checkpoint_dir
+      call fatal_error('TF_save_checkpoint', 'training is not available with gpu_mi355x')
+    endsubroutine TF_save_checkpoint
+!***********************************************************************
+endmodule GPU

@@ -1,0 +1,252 @@
+"""Python host mirror of the Pencil Code ``module GPU`` interface over libpcmx.
+
+The reference's Fortran module (src/gpu.h, src/gpu_astaroth.f90) exposes
+register_GPU, initialize_GPU, load_farray_to_GPU, before_boundary_gpu,
+rhs_GPU, update_after_substep_gpu, copy_farray_from_GPU,
+gpu_prepare_for_first_substep, get_gpu_reduced_vars, update_on_gpu,
+test_gpu_bcs, split_update_gpu and finalize_GPU.  This module exposes the
+same names with the same argument meaning over the C ABI of
+``include/pcmx.h`` (loaded with ctypes), so Python drivers and the tests
+read like the reference's call sequence in equ.f90 / timestep.f90.
+
+The f-array is the reference layout ``f(mx,my,mz,mfarray)`` in Fortran
+order, i.e. a C-ordered numpy array of shape ``(mfarray, mz, my, mx)``.
+
+There is no CPU fallback: if libpcmx.so is missing or no GPU is present,
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpcmx.so")
+NGHOST = 3
+N_GPU_REDUCED_VARS = 20
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+KERNEL_AUTO, KERNEL_DIRECT, KERNEL_ZMARCH = 0, 1, 2
+
+
+class PcmxError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("mvar", ctypes.c_int),
+        ("iux", ctypes.c_int), ("ilnrho", ctypes.c_int), ("iss", ctypes.c_int), ("iax", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("chi", ctypes.c_double),
+        ("cs0", ctypes.c_double), ("gamma", ctypes.c_double), ("cp", ctypes.c_double),
+        ("rho0", ctypes.c_double), ("mu0", ctypes.c_double),
+        ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
+        ("ldt", ctypes.c_int), ("precision", ctypes.c_int), ("kernel", ctypes.c_int),
+    ]
+
+
+def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
+                cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
+                kernel=None, L=2 * math.pi):
+    """Parameters of one periodic box with the reference defaults
+    (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
+    registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
+    kernel None takes PCMX_KERNEL from the environment (0 auto, 1 direct,
+    2 z-marching), default auto."""
+    if kernel is None:
+        kernel = int(os.environ.get("PCMX_KERNEL", KERNEL_AUTO))
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    iss = 4 if lentropy else -1
+    iax = (5 if lentropy else 4) if lmagnetic else -1
+    mvar = 4 + (1 if lentropy else 0) + (3 if lmagnetic else 0)
+    return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, Lz / nz, nu, eta, chi, cs0, gamma,
+                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel)
+
+
+_lib = None
+
+
+def lib():
+    """Load libpcmx.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PcmxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.pcmx_last_error.restype = ctypes.c_char_p
+        L.pcmx_last_error.argtypes = []
+        sig = {
+            "pcmx_register": [],
+            "pcmx_initialize": [dp, ctypes.c_int, ctypes.POINTER(Params), ctypes.c_double, ctypes.c_int],
+            "pcmx_finalize": [],
+            "pcmx_load_farray": [dp, ctypes.c_int],
+            "pcmx_copy_farray": [dp, ctypes.c_int],
+            "pcmx_before_boundary": [ctypes.c_int, ctypes.c_int, ctypes.c_double],
+            "pcmx_rhs": [ctypes.c_int, ctypes.c_double, dp],
+            "pcmx_update_after_substep": [],
+            "pcmx_prepare_for_first_substep": [ctypes.c_double, dp],
+            "pcmx_get_reduced_vars": [dp],
+            "pcmx_update_by_name": [ctypes.c_char_p, ctypes.c_double],
+            "pcmx_update_by_index": [ctypes.c_int, ctypes.c_double],
+            "pcmx_test_bcs": [dp],
+            "pcmx_split_update": [],
+            "pcmx_time_step": [ctypes.c_int, dp, dp, ctypes.c_int],
+            "pcmx_kernel_timing": [dp],
+            "pcmx_set_timing": [ctypes.c_int],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise PcmxError(lib().pcmx_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise PcmxError("f-array must be a C-contiguous float64 array of shape (mfarray, mz, my, mx)")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ------------------------------------------------------------------ module GPU
+def register_gpu():
+    _check(lib().pcmx_register())
+
+
+def initialize_gpu(f, par, t=0.0, nt=1):
+    """initialize_GPU(f): allocate device state; load f when nt > 0."""
+    if f is not None:
+        _check(lib().pcmx_initialize(_ptr(f), f.shape[0], ctypes.byref(par), t, nt))
+    else:
+        _check(lib().pcmx_initialize(None, 0, ctypes.byref(par), t, 0))
+
+
+def finalize_gpu():
+    _check(lib().pcmx_finalize())
+
+
+def load_farray_to_gpu(f):
+    _check(lib().pcmx_load_farray(_ptr(f), f.shape[0]))
+
+
+def copy_farray_from_gpu(f):
+    _check(lib().pcmx_copy_farray(_ptr(f), f.shape[0]))
+
+
+def before_boundary_gpu(lrmv, isubstep, t):
+    _check(lib().pcmx_before_boundary(int(lrmv), isubstep, t))
+
+
+def rhs_gpu(isubstep, t, dt):
+    """rhs_GPU: one RK substep.  Returns dt (the Courant step on substep 1
+    of a Courant-limited run, else the dt passed in)."""
+    d = ctypes.c_double(dt)
+    _check(lib().pcmx_rhs(isubstep, t, ctypes.byref(d)))
+    return d.value
+
+
+def update_after_substep_gpu():
+    _check(lib().pcmx_update_after_substep())
+
+
+def split_update_gpu():
+    _check(lib().pcmx_split_update())
+
+
+def gpu_prepare_for_first_substep(t=0.0):
+    d = ctypes.c_double(0.0)
+    _check(lib().pcmx_prepare_for_first_substep(t, ctypes.byref(d)))
+    return d.value
+
+
+def get_gpu_reduced_vars():
+    dst = np.zeros(N_GPU_REDUCED_VARS)
+    _check(lib().pcmx_get_reduced_vars(_ptr(dst)))
+    return dst
+
+
+def update_on_gpu(index, varname=None, value=None):
+    """update_on_gpu(index, varname, value): index < 0 looks the name up and
+    returns its index; an unknown name raises (the reference: fatal_error)."""
+    if index >= 0:
+        _check(lib().pcmx_update_by_index(index, value))
+        return index
+    return _check(lib().pcmx_update_by_name(varname.encode(), value))
+
+
+def test_gpu_bcs():
+    d = ctypes.c_double(-1.0)
+    _check(lib().pcmx_test_bcs(ctypes.byref(d)))
+    return d.value
+
+
+def time_step(nsteps, t, dt, ldiag=True):
+    """nsteps full RK3 steps on the device.  Returns (t, dt)."""
+    tt, dd = ctypes.c_double(t), ctypes.c_double(dt)
+    _check(lib().pcmx_time_step(nsteps, ctypes.byref(tt), ctypes.byref(dd), int(ldiag)))
+    return tt.value, dd.value
+
+
+def set_timing(enable):
+    _check(lib().pcmx_set_timing(int(enable)))
+
+
+def kernel_timing():
+    out = np.zeros(2)
+    _check(lib().pcmx_kernel_timing(_ptr(out)))
+    return float(out[0]), int(out[1])
+
+
+# ------------------------------------------------------------------ helpers
+def farray_from_interior(interior, mfarray=None):
+    """(mvar, nz, ny, nx) -> zero-ghost f-array (mfarray, mz, my, mx)."""
+    interior = np.asarray(interior, dtype=np.float64)
+    mv, nz, ny, nx = interior.shape
+    mf = mfarray or mv
+    f = np.zeros((mf, nz + 2 * NGHOST, ny + 2 * NGHOST, nx + 2 * NGHOST))
+    g = NGHOST
+    f[:mv, g:g + nz, g:g + ny, g:g + nx] = interior
+    return f
+
+
+def interior_of(f, mvar):
+    g = NGHOST
+    return f[:mvar, g:-g, g:-g, g:-g]
+
+
+def advance(interior, par, nsteps, dt, t=0.0, per_substep=False):
+    """Convenience driver in the reference's call order: load, nsteps steps,
+    copy back.  Returns (interior_after, t, dt)."""
+    f = farray_from_interior(interior)
+    initialize_gpu(f, par, t=t, nt=1)
+    try:
+        if per_substep:
+            for _ in range(nsteps):
+                ds = 0.0
+                for isub in (1, 2, 3):
+                    before_boundary_gpu(False, isub, t)
+                    dt = rhs_gpu(isub, t, dt)
+                    ds = 1.0 if isub == 1 else (-5.0 / 9.0 if isub == 2 else -153.0 / 128.0) * ds + 1.0
+                    t = t + ds * (dt * (1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0)[isub - 1])
+                    update_after_substep_gpu()
+        else:
+            t, dt = time_step(nsteps, t, dt)
+        out = np.zeros_like(f)
+        copy_farray_from_gpu(out)
+        return interior_of(out, par.mvar).copy(), t, dt
+    finally:
+        finalize_gpu()

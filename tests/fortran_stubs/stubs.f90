@@ -1,0 +1,65 @@
+! Interface stubs for compile-checking pencil-code_amd/fortran/gpu_mi355x.f90
+! outside a Pencil Code tree.  They declare only the names that module uses
+! (types and ranks as in the reference's Cdata/Cparam/Messages/Sub/...),
+! with no behaviour; the check is a compile, never a run.
+module Cparam
+  implicit none
+  integer, parameter :: ikind8=selected_int_kind(14), rkind8=selected_real_kind(12)
+  integer, parameter :: nx=32, ny=32, nz=32, mx=nx+6, my=ny+6, mz=nz+6
+  integer, parameter :: mvar=7, mfarray=7, ncpus=1
+  logical, parameter :: lhydro=.true., ldensity=.true., lmagnetic=.true., lentropy=.false.
+  logical, parameter :: ltemperature=.false., lthermal_energy=.false.
+end module Cparam
+module Cdata
+  use Cparam
+  implicit none
+  real(kind=rkind8) :: t=0.
+  real :: dt=0., dx=1., dy=1., dz=1., cdt=0.9, cdtv=0.25, ddt=0., mu0=1.
+  logical :: ldt=.true., lroot=.true., lcartesian_coords=.true., ldensity_nolog=.false.
+  logical :: loutput_varn_at_exact_tsnap=.false.
+  logical, dimension(3) :: lperi=.true., lequidist=.true.
+  integer :: iux=1, ilnrho=4, iss=0, iax=5, itorder=3, nt=10
+end module Cdata
+module General
+  implicit none
+  interface keep_compiler_quiet
+    module procedure kcq_r4, kcq_i8
+  end interface
+contains
+  subroutine kcq_r4(a)
+    real, dimension(:,:,:,:) :: a
+  end subroutine
+  subroutine kcq_i8(a)
+    integer(kind=selected_int_kind(14)) :: a
+  end subroutine
+end module General
+module Messages
+  implicit none
+contains
+  subroutine fatal_error(location, message)
+    character(len=*) :: location, message
+  end subroutine
+end module Messages
+module SharedVariables
+  implicit none
+contains
+  subroutine get_shared_variable(name, p)
+    character(len=*) :: name
+    real, pointer :: p
+  end subroutine
+end module SharedVariables
+module EquationOfState
+  implicit none
+  real :: rho0=1.
+end module EquationOfState
+module Sub
+  implicit none
+contains
+  subroutine set_dt(dt1_)
+    real :: dt1_
+  end subroutine
+end module Sub
+module File_io
+  implicit none
+  integer :: parallel_unit=10
+end module File_io
