@@ -105,6 +105,7 @@ void pco_fill_ghosts(double* f, int nx, int ny, int nz, int nvar) {
   for (int k = 0; k < nvar; ++k) {
     double* a = f + k * mw;
     /* x: f(1:l1-1)=f(l2i:l2), f(l2+1:mx)=f(l1:l1i), all m, n */
+#pragma omp parallel for schedule(static)
     for (int iz = 0; iz < mz; ++iz)
       for (int iy = 0; iy < my; ++iy) {
         double* row = a + ((long)iz * my + iy) * mx;
@@ -114,6 +115,7 @@ void pco_fill_ghosts(double* f, int nx, int ny, int nz, int nvar) {
         }
       }
     /* y over full x extent */
+#pragma omp parallel for schedule(static)
     for (int iz = 0; iz < mz; ++iz)
       for (int g = 0; g < NG; ++g) {
         memcpy(a + ((long)iz * my + g) * mx, a + ((long)iz * my + ny + g) * mx, sizeof(double) * mx);
@@ -685,6 +687,9 @@ void pco_advance(double* f, double* df, const pco_params* p, int nsteps, double*
   const int nvar = pco_nvar(p);
   const int ldt = (p->dt == 0.0);
   double dt_ = ldt ? *dt : p->dt;
+#ifdef _OPENMP
+  const int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
   for (int it = 0; it < nsteps; ++it) {
     double dt_beta_ts[3];
     if (!ldt)
@@ -698,6 +703,7 @@ void pco_advance(double* f, double* df, const pco_params* p, int nsteps, double*
         ds = 0.0;
       } else {
         const double a = ALPHA_TS[itsub];
+#pragma omp parallel for schedule(static) num_threads(nth)
         for (long q = 0; q < ntot; ++q) df[q] = a * df[q];
         ds = a * ds;
       }
@@ -710,6 +716,7 @@ void pco_advance(double* f, double* df, const pco_params* p, int nsteps, double*
         for (int s = 0; s < 3; ++s) dt_beta_ts[s] = dt_ * BETA_TS[s];
       const double dtsub = ds * dt_beta_ts[itsub];
       const double db = dt_beta_ts[itsub];
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nth)
       for (int k = 0; k < nvar; ++k)
         for (int n = NG; n < NG + g.nz; ++n)
           for (int m = NG; m < NG + g.ny; ++m) {
