@@ -62,12 +62,36 @@ def run_form(n, nvar_mag, precision, steps, warmup, dt, kernel, f):
                 achieved=achieved, urms=float(red[0]), brms=float(red[2]))
 
 
+def copy_calibration(mib, reps):
+    """Device-to-device copies of a known size (the runtime's blit kernel),
+    run beside the RHS kernel under rocprofv3 --pmc so that FETCH_SIZE and
+    WRITE_SIZE can be scaled to bytes: each copy reads and writes mib MiB."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed in copy calibration")
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(n))
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed in copy calibration")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    log(f"bench: copy calibration {reps} x {mib} MiB device-to-device")
+
+
 def cpu_baseline(threads):
     """Oracle (C port of the reference pencil loop) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    n, steps = 256, 4
+    n, steps = 256, 30
     p = O.make_params(n, lmagnetic=True, nu=1e-3, eta=1e-3, dt=1e-3)
     f0 = pcmx_ics.config_state("noise", n, 7, lmagnetic=True)
     O.advance(f0, p, 1, nthreads=threads)          # warm caches / threads
@@ -89,6 +113,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=None)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="also run 4 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -110,6 +136,8 @@ def main():
         rmx = run_form(n, True, pcmx.PREC_MIXED, args.steps, args.warmup, args.dt, args.kernel, f)
         log(f"bench: mixed {rmx}")
     del f
+    if args.copy_calib:
+        copy_calibration(args.copy_calib, 4)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

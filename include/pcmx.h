@@ -55,7 +55,8 @@ typedef struct pcmx_params {
   double cdt, cdtv;          /* run_pars Courant factors (cdata.f90:235-239) */
   int ldt;                   /* 1: Courant-limited dt (run_pars dt=0, timestep.f90:67) */
   int precision;             /* PCMX_PREC_* */
-  int kernel;                /* 0: auto, 1: direct (one thread per cell), 2: z-marching LDS */
+  int kernel;                /* 0: auto, 1: direct (one thread per cell), 2: z-marching LDS,
+                                3: z-marching LDS with flow/field halves on separate waves */
 } pcmx_params;
 
 /* Last error message (thread-local to the calling thread). */

@@ -90,27 +90,21 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
     for (int i = 0; i < 3; ++i) {
       d.u[i] = (T)fc[i];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        d.uij[i][j] = d1(i, j);
-        d.d2u[i][j] = d2(i, j);
-      }
+      for (int j = 0; j < 3; ++j) d.uij[i][j] = d1(i, j);
+      // D_xy u_i feeds graddiv for i = x, y; D_xz for i = x, z; D_yz for i = y, z
+      const T mxy = (i != 2) ? mix(i, 0) : (T)0;
+      const T mxz = (i != 1) ? mix(i, 1) : (T)0;
+      const T myz = (i != 0) ? mix(i, 2) : (T)0;
+      add_u_second(d, i, d2(i, 0), d2(i, 1), d2(i, 2), mxy, mxz, myz);
       d.glnrho[i] = d1(3, i);
     }
     d.lnrho = (T)fc[3];
-    d.mxu[0][0] = mix(0, 0);
-    d.mxu[0][1] = mix(1, 0);
-    d.mxu[1][0] = mix(0, 1);
-    d.mxu[1][2] = mix(2, 1);
-    d.mxu[2][1] = mix(1, 2);
-    d.mxu[2][2] = mix(2, 2);
     if constexpr (ENT) {
       d.ss = (T)fc[S::ISS];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        d.gss[j] = d1(S::ISS, j);
-        d.d2ss[j] = d2(S::ISS, j);
-        d.d2lnrho[j] = d2(3, j);
-      }
+      for (int j = 0; j < 3; ++j) d.gss[j] = d1(S::ISS, j);
+      d.del2ss = d2(S::ISS, 0) + d2(S::ISS, 1) + d2(S::ISS, 2);
+      d.del2lnrho = d2(3, 0) + d2(3, 1) + d2(3, 2);
     }
     if constexpr (MAG) {
       constexpr int IA = S::IAX;

@@ -23,6 +23,7 @@
 #include "pcmx_direct.h"
 #include "pcmx_kernels.h"
 #include "pcmx_zmarch.h"
+#include "pcmx_zsplit.h"
 
 using namespace pcmx;
 
@@ -90,7 +91,7 @@ int check_params(const pcmx_params* p) {
     return fail(PCMX_ERR_ARG, "unknown precision %d", p->precision);
   if (ent && p->gamma == 1.0) return fail(PCMX_ERR_ARG, "entropy needs gamma /= 1");
   if (p->cdt <= 0 || p->cdtv <= 0) return fail(PCMX_ERR_ARG, "cdt and cdtv must be positive");
-  if (p->kernel < 0 || p->kernel > 2) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
+  if (p->kernel < 0 || p->kernel > 3) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
   return PCMX_OK;
 }
 
@@ -140,9 +141,25 @@ void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
 }
 long direct_blocks(const pcmx_params& p) { return (long)((p.nx + 63) / 64) * ((p.ny + 3) / 4) * p.nz; }
 
+// RK stage of a launch of the split z-marching kernel
+int stage_of(const KArgs& a) {
+  if (a.mode == MODE_DT) return ST_DT;
+  if (a.mode == MODE_W) return ST_W;
+  if (a.first) return ST_S1;
+  return a.write_w ? ST_S2 : ST_S3;
+}
+
 template <typename T, bool MAG, bool ENT>
 void launch_t(const Engine& e, const KArgs& a, bool dt) {
-  if (e.kernel == 2) {
+  if (e.kernel == 3) {
+    switch (stage_of(a)) {
+      case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1>(a, e.phys, e.zplan, e.stream); break;
+      case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2>(a, e.phys, e.zplan, e.stream); break;
+      case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3>(a, e.phys, e.zplan, e.stream); break;
+      case ST_W: launch_zsplit<T, MAG, ENT, ST_W>(a, e.phys, e.zplan, e.stream); break;
+      default: launch_zsplit<T, MAG, ENT, ST_DT>(a, e.phys, e.zplan, e.stream); break;
+    }
+  } else if (e.kernel == 2) {
     if (dt) launch_zmarch<T, MAG, ENT, true>(a, e.phys, e.zplan, e.stream);
     else launch_zmarch<T, MAG, ENT, false>(a, e.phys, e.zplan, e.stream);
   } else {
@@ -162,7 +179,7 @@ void launch_p(const Engine& e, const KArgs& a, bool dt) {
   }
 }
 
-long nblocks(const Engine& e) { return e.kernel == 2 ? e.zplan.nblocks : direct_blocks(e.par); }
+long nblocks(const Engine& e) { return e.kernel >= 2 ? e.zplan.nblocks : direct_blocks(e.par); }
 
 int launch_rhs(Engine& e, const KArgs& a, bool dt) {
   if (e.par.precision == PCMX_PREC_MIXED) launch_p<float>(e, a, dt);
@@ -270,15 +287,15 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   e->nvar = par->mvar;
   e->mag = par->iax >= 0;
   e->ent = par->iss >= 0;
-  // kernel choice: the z-marching LDS kernel needs whole tiles in x and y
+  // kernel choice: the z-marching LDS kernels need whole tiles in x and y
   int kern = par->kernel;
-  if (kern == 0) kern = zmarch_supported(*par) ? 2 : 1;
-  if (kern == 2 && !zmarch_supported(*par)) {
+  if (kern == 0) kern = zmarch_supported(*par) ? 3 : 1;
+  if (kern >= 2 && !zmarch_supported(*par)) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
   }
   e->kernel = kern;
-  if (kern == 2) e->zplan = zmarch_plan(*par);
+  if (kern >= 2) e->zplan = zmarch_plan(*par);
   const size_t bytes = sizeof(double) * e->nxyz * e->nvar;
   auto alloc = [&](double** p, size_t n) -> int {
     hipError_t err = hipMalloc((void**)p, n);
@@ -286,7 +303,7 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
       return fail(PCMX_ERR_HIP, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(err));
     return PCMX_OK;
   };
-  e->partials_cap = std::max(direct_blocks(*par), e->kernel == 2 ? e->zplan.nblocks : 0L);
+  e->partials_cap = std::max(direct_blocks(*par), e->kernel >= 2 ? e->zplan.nblocks : 0L);
   if ((rc = alloc(&e->f[0], bytes)) || (rc = alloc(&e->f[1], bytes)) || (rc = alloc(&e->w, bytes)) ||
       (rc = alloc(&e->partials, sizeof(double) * NRED * e->partials_cap)) ||
       (rc = alloc(&e->results, sizeof(double) * R_COUNT))) {

@@ -39,20 +39,46 @@ struct Phys {
 #define PCMX_D2C2 (-27.0 / 180.0)
 #define PCMX_D2C3 (2.0 / 180.0)
 
-// Derivatives at one cell.  mix?[pair][c]: pair 0 = xy, 1 = xz, 2 = yz,
-// component c of the vector; only the entries the RHS uses are filled.
+// Derivatives at one cell.  The second derivatives of uu, lnrho and ss enter
+// the RHS only through del2 and graddiv (del2v_etc, sub.f90:2681-2701), so
+// they are kept as those sums; a producer adds the terms in the reference's
+// order: del2_i = (D_xx + D_yy) + D_zz of component i, and graddiv_i =
+// sum over j = x, y, z of D_j D_i u_j, the u_x term first (so producers visit
+// u_x, u_y, u_z in that order; the first term initialises the sum).
+// mxa[pair][c]: pair 0 = xy, 1 = xz, 2 = yz, component c of A; only the
+// entries the curl of B uses are filled.
 template <typename T>
 struct Derivs {
   T u[3], lnrho, ss, a[3];
   T uij[3][3];        // uij[i][j] = D_j u_i           (gij, sub.f90:1505)
   T glnrho[3], gss[3];
   T aij[3][3];        // D_j A_i
-  T d2u[3][3];        // D2_j u_i                      (del2v_etc fijj)
-  T d2lnrho[3], d2ss[3];
+  T del2u[3], gdivu[3];
+  T del2lnrho, del2ss;
   T d2a[3][3];        // D2_j A_i                      (gij_etc d2A(j,j,i))
-  T mxu[3][3];        // mixed of u: [0][0..1] xy of u_x,u_y; [1][0],[1][2] xz; [2][1],[2][2] yz
-  T mxa[3][3];        // same for A
+  T mxa[3][3];
 };
+
+// Adds the second derivatives of component i of uu to del2u and gdivu.
+// mixed[] holds D_xy, D_xz, D_yz of u_i (entries not used by graddiv are
+// ignored).  Must be called for i = 0, 1, 2 in that order.
+template <typename T>
+__device__ __forceinline__ void add_u_second(Derivs<T>& d, int i, T dxx, T dyy, T dzz, T mxy, T mxz, T myz) {
+  d.del2u[i] = dxx + dyy + dzz;
+  if (i == 0) {
+    d.gdivu[0] = dxx;
+    d.gdivu[1] = mxy;
+    d.gdivu[2] = mxz;
+  } else if (i == 1) {
+    d.gdivu[0] = d.gdivu[0] + mxy;
+    d.gdivu[1] = d.gdivu[1] + dyy;
+    d.gdivu[2] = d.gdivu[2] + myz;
+  } else {
+    d.gdivu[0] = d.gdivu[0] + mxz;
+    d.gdivu[1] = d.gdivu[1] + myz;
+    d.gdivu[2] = d.gdivu[2] + dzz;
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ T pexp(T x);
@@ -81,13 +107,82 @@ __device__ __forceinline__ T d_sij2(const T (&s)[3][3]) {
   return b;
 }
 
-// Adds the RHS to w[] (already alpha-scaled) and returns the cell's
-// dt1_max_loc in *dt1 (only meaningful when DT), u2 and b2 for diagnostics.
-// NV slots: 0..2 uu, 3 lnrho, [4 ss], then aa.
-template <typename T, bool MAG, bool ENT, bool DT>
-__device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, double* w, int iss,
-                                          int iax, double* dt1, double* u2_out, double* b2_out) {
-  const T dx_1 = (T)P.dx_1, dy_1 = (T)P.dy_1, dz_1 = (T)P.dz_1;
+// ---------------------------------------------------------------------------
+// The RHS is split in two halves so that a kernel may evaluate them on
+// different threads (the field half and the flow half exchange four numbers
+// per cell): field_point covers src/magnetic.f90 (B, J, J x B / rho,
+// induction, Alfven speed), flow_point_pre and flow_point_post the rest.
+// rhs_point runs the three in sequence; every kernel therefore executes the
+// same arithmetic in the same order.
+
+// What the field half hands to the flow half.
+template <typename T>
+struct FieldOut {
+  T jxbr[3];    // J x B / rho                         magnetic.f90:4871-4888
+  T j2;         // J^2 (ohmic heating, entropy runs)   magnetic.f90:6143
+  T advec_va2;  // Alfven Courant term                 magnetic.f90:5180
+};
+
+// Pencils the flow half keeps between its two parts.
+template <typename T>
+struct FlowKeep {
+  T rho1, TT1, cs2, advec_uu;
+};
+
+// Magnetic pencils (magnetic.f90:4279-5225; sub.f90:2945-3113) and daa_dt
+// (magnetic.f90:6247,6524).  d.a*, d.aij, d.d2a, d.mxa and d.u are read;
+// wa[0..2] (already alpha-scaled) receive the induction RHS.
+template <typename T, bool ENT, bool DT>
+__device__ __forceinline__ void field_point(const Derivs<T>& d, T rho1, const Phys& P, double* wa,
+                                            FieldOut<T>& o, double* b2_out) {
+  const T u0 = d.u[0], u1 = d.u[1], u2v = d.u[2];
+  T bb[3];
+  bb[0] = d.aij[2][1] - d.aij[1][2];                                  // curl_mn
+  bb[1] = d.aij[0][2] - d.aij[2][0];
+  bb[2] = d.aij[1][0] - d.aij[0][1];
+  const T b2 = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
+  T uxb[3];
+  uxb[0] = u1 * bb[2] - u2v * bb[1];                                  // cross_mn
+  uxb[1] = u2v * bb[0] - u0 * bb[2];
+  uxb[2] = u0 * bb[1] - u1 * bb[0];
+  // curlb = curl_mn(bij) with bij(i,j) = eps_ikl d2A(k,j,l)
+  T curlb[3];
+  curlb[0] = (d.mxa[0][1] - d.d2a[0][1]) - (d.d2a[0][2] - d.mxa[1][2]);
+  curlb[1] = (d.mxa[2][2] - d.d2a[1][2]) - (d.d2a[1][0] - d.mxa[0][0]);
+  curlb[2] = (d.mxa[1][0] - d.d2a[2][0]) - (d.d2a[2][1] - d.mxa[2][1]);
+  T del2a[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) del2a[i] = d.d2a[i][0] + d.d2a[i][1] + d.d2a[i][2];
+  T jj[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) jj[i] = (T)P.mu01 * curlb[i];
+  o.j2 = 0;
+  if constexpr (ENT) o.j2 = jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2];
+  T jxb[3];
+  jxb[0] = jj[1] * bb[2] - jj[2] * bb[1];
+  jxb[1] = jj[2] * bb[0] - jj[0] * bb[2];
+  jxb[2] = jj[0] * bb[1] - jj[1] * bb[0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o.jxbr[i] = rho1 * jxb[i];
+  o.advec_va2 = 0;
+  if constexpr (DT) {
+    const T b1x = bb[0] * (T)P.dx_1, b1y = bb[1] * (T)P.dy_1, b1z = bb[2] * (T)P.dz_1;
+    o.advec_va2 = (b1x * b1x + b1y * b1y + b1z * b1z) * (T)P.mu01 * rho1;  // :5180
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) wa[j] = wa[j] + (double)(uxb[j] + (T)P.eta * del2a[j]);  // :6247,6524
+  *b2_out = (double)b2;
+}
+
+// Hydro, density, EOS, energy and viscosity pencils and their RHS terms up to
+// (not including) the magnetic contributions.  wf[0..2] uu, wf[3] lnrho,
+// wf[4] ss (ENT), already alpha-scaled.  rho1 = exp(-lnrho) is taken as an
+// argument so that both halves use the same value.  Without entropy the EOS
+// is isothermal (gamma = 1) or, with POLY, polytropic (noentropy.f90:94-103);
+// it is a compile-time choice so that no branch splits the algebra.
+template <typename T, bool ENT, bool DT, bool POLY>
+__device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const Phys& P, double* wf,
+                                               FlowKeep<T>& k, double* u2_out) {
   // ---- hydro pencils (hydro.f90:3925-4000; sub.f90 helpers) ----
   const T u0 = d.u[0], u1 = d.u[1], u2v = d.u[2];
   const T u2 = u0 * u0 + u1 * u1 + u2v * u2v;                           // dot2_mn
@@ -106,16 +201,11 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
   T ugu[3];                                                             // u_dot_grad_vec
 #pragma unroll
   for (int j = 0; j < 3; ++j) ugu[j] = u0 * d.uij[j][0] + u1 * d.uij[j][1] + u2v * d.uij[j][2];
-  T del2u[3], gdivu[3];                                                 // del2v_etc
-#pragma unroll
-  for (int i = 0; i < 3; ++i) del2u[i] = d.d2u[i][0] + d.d2u[i][1] + d.d2u[i][2];
-  gdivu[0] = d.d2u[0][0] + d.mxu[0][1] + d.mxu[1][2];
-  gdivu[1] = d.mxu[0][0] + d.d2u[1][1] + d.mxu[2][2];
-  gdivu[2] = d.mxu[1][0] + d.mxu[2][1] + d.d2u[2][2];
+  const T* del2u = d.del2u;                                             // del2v_etc
+  const T* gdivu = d.gdivu;
 
   // ---- density pencils (density.f90:2727-2811) ----
   const T lnrho = d.lnrho;
-  const T rho1 = pexp<T>(-lnrho);
   const T uglnrho = u0 * d.glnrho[0] + u1 * d.glnrho[1] + u2v * d.glnrho[2];
   T sglnrho[3];                                                         // multmv_mn
 #pragma unroll
@@ -126,14 +216,14 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
   T cs2, TT1 = 0, glnTT[3] = {0, 0, 0}, del2lnTT = 0, ugss = 0;
   T fpres[3];
   if constexpr (!ENT) {
-    if (P.polytropic) cs2 = (T)P.cs20 * pexp<T>((T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
+    if constexpr (POLY) cs2 = (T)P.cs20 * pexp<T>((T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
     else cs2 = (T)P.cs20;                                               // eos_idealgas.f90:969-1000
 #pragma unroll
     for (int j = 0; j < 3; ++j) fpres[j] = -cs2 * d.glnrho[j];          // noentropy.f90:252-300
   } else {
     const T ss = d.ss;
-    const T del2lnrho = d.d2lnrho[0] + d.d2lnrho[1] + d.d2lnrho[2];
-    const T del2ss = d.d2ss[0] + d.d2ss[1] + d.d2ss[2];
+    const T del2lnrho = d.del2lnrho;
+    const T del2ss = d.del2ss;
     const T dl = lnrho - (T)P.lnrho0;
     cs2 = (T)P.cs20 * pexp<T>((T)P.cv1 * ss + (T)P.gamma_m1 * dl);       // eos_idealgas.f90:871
     const T lnTT = (T)P.lnTT0 + (T)P.cv1 * ss + (T)P.gamma_m1 * dl;      // :876
@@ -153,76 +243,53 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
   for (int j = 0; j < 3; ++j)
     fvisc[j] = (T)P.nu * (del2u[j] + (T)2. * sglnrho[j] + (T)(1. / 3.) * gdivu[j]);
 
-  // ---- magnetic pencils (magnetic.f90:4279-5225; sub.f90:2945-3113) ----
-  T bb[3] = {0, 0, 0}, jxbr[3] = {0, 0, 0}, uxb[3] = {0, 0, 0}, del2a[3] = {0, 0, 0};
-  T b2 = 0, j2 = 0, advec_va2 = 0;
-  if constexpr (MAG) {
-    bb[0] = d.aij[2][1] - d.aij[1][2];                                  // curl_mn
-    bb[1] = d.aij[0][2] - d.aij[2][0];
-    bb[2] = d.aij[1][0] - d.aij[0][1];
-    b2 = bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2];
-    uxb[0] = u1 * bb[2] - u2v * bb[1];                                  // cross_mn
-    uxb[1] = u2v * bb[0] - u0 * bb[2];
-    uxb[2] = u0 * bb[1] - u1 * bb[0];
-    // curlb = curl_mn(bij) with bij(i,j) = eps_ikl d2A(k,j,l)
-    T curlb[3];
-    curlb[0] = (d.mxa[0][1] - d.d2a[0][1]) - (d.d2a[0][2] - d.mxa[1][2]);
-    curlb[1] = (d.mxa[2][2] - d.d2a[1][2]) - (d.d2a[1][0] - d.mxa[0][0]);
-    curlb[2] = (d.mxa[1][0] - d.d2a[2][0]) - (d.d2a[2][1] - d.mxa[2][1]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) del2a[i] = d.d2a[i][0] + d.d2a[i][1] + d.d2a[i][2];
-    T jj[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) jj[i] = (T)P.mu01 * curlb[i];
-    if constexpr (ENT) j2 = jj[0] * jj[0] + jj[1] * jj[1] + jj[2] * jj[2];
-    T jxb[3];
-    jxb[0] = jj[1] * bb[2] - jj[2] * bb[1];
-    jxb[1] = jj[2] * bb[0] - jj[0] * bb[2];
-    jxb[2] = jj[0] * bb[1] - jj[1] * bb[0];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) jxbr[i] = rho1 * jxb[i];
-    if constexpr (DT) {
-      const T b1x = bb[0] * dx_1, b1y = bb[1] * dy_1, b1z = bb[2] * dz_1;
-      advec_va2 = (b1x * b1x + b1y * b1y + b1z * b1z) * (T)P.mu01 * rho1;  // :5180
-    }
-  }
-
-  // ---- calc_all_rhs: duu_dt, dlnrho_dt, denergy_dt, daa_dt ----
+  // ---- calc_all_rhs: duu_dt, dlnrho_dt, denergy_dt (equ.f90:1268-1274) ----
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    double acc = w[j];
+    double acc = wf[j];
     acc = acc - (double)ugu[j];        // hydro.f90:4384
     acc = acc + (double)fvisc[j];      // viscosity.f90:2823
     acc = acc + (double)fpres[j];      // noentropy.f90:381 / entropy.f90:3714
-    if constexpr (MAG) acc = acc + (double)jxbr[j];  // magnetic.f90:5531
-    w[j] = acc;
+    wf[j] = acc;
   }
   {
     const T density_rhs = -divu - uglnrho;  // density.f90:2929-2930
-    w[3] = w[3] + (double)density_rhs;
+    wf[3] = wf[3] + (double)density_rhs;
   }
   if constexpr (ENT) {
     const T visc_heat = (T)2 * (T)P.nu * d_sij2(s);
-    double acc = w[iss];
+    double acc = wf[4];
     acc = acc - (double)ugss;                               // entropy.f90:3758
     acc = acc + (double)(TT1 * visc_heat);                  // viscosity.f90:2764
     const T a0 = d.glnrho[0] + glnTT[0], a1 = d.glnrho[1] + glnTT[1], a2 = d.glnrho[2] + glnTT[2];
     const T g2 = a0 * glnTT[0] + a1 * glnTT[1] + a2 * glnTT[2];
     acc = acc + (double)((T)P.chi * (T)P.cp * (del2lnTT + g2));  // entropy.f90:4852-4947
-    if constexpr (MAG) acc = acc + (double)((T)P.eta * (T)P.mu0 * j2 * rho1 * TT1);  // magnetic.f90:6143
-    w[iss] = acc;
+    wf[4] = acc;
   }
+  k.rho1 = rho1;
+  k.TT1 = TT1;
+  k.cs2 = cs2;
+  k.advec_uu = 0;
+  if constexpr (DT)
+    k.advec_uu = fabs(u0) * (T)P.dx_1 + fabs(u1) * (T)P.dy_1 + fabs(u2v) * (T)P.dz_1;  // hydro.f90:3500
+  *u2_out = (double)u2;
+}
+
+// The magnetic contributions to the flow RHS (Lorentz force last in duu_dt,
+// magnetic.f90:5531; ohmic heating last in the entropy equation,
+// magnetic.f90:6143) and set_dt1_max (equ.f90:1858-1912).
+template <typename T, bool MAG, bool ENT, bool DT>
+__device__ __forceinline__ void flow_point_post(const FlowKeep<T>& k, const FieldOut<T>& fo, const Phys& P,
+                                                double* wf, double* dt1) {
   if constexpr (MAG) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) w[iax + j] = w[iax + j] + (double)(uxb[j] + (T)P.eta * del2a[j]);  // :6247,6524
+    for (int j = 0; j < 3; ++j) wf[j] = wf[j] + (double)fo.jxbr[j];
+    if constexpr (ENT) wf[4] = wf[4] + (double)((T)P.eta * (T)P.mu0 * fo.j2 * k.rho1 * k.TT1);
   }
-
-  // ---- set_dt1_max (equ.f90:1858-1912) ----
   if constexpr (DT) {
-    const T advec_uu = fabs(u0) * dx_1 + fabs(u1) * dy_1 + fabs(u2v) * dz_1;  // hydro.f90:3500
-    const T advec_cs2 = cs2 * (T)P.dxyz_2;
-    const T advec2 = advec_va2 + advec_cs2;
-    const T maxadvec = advec_uu + psqrt<T>(advec2);
+    const T advec_cs2 = k.cs2 * (T)P.dxyz_2;
+    const T advec2 = (MAG ? fo.advec_va2 : (T)0) + advec_cs2;
+    const T maxadvec = k.advec_uu + psqrt<T>(advec2);
     T maxdiffus = (T)P.nu * (T)P.dxyz_2;                                 // viscosity.f90:2435
     if constexpr (MAG) maxdiffus = fmax(maxdiffus, (T)P.eta * (T)P.dxyz_2);
     if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
@@ -230,8 +297,28 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
     const T dt1_diffus = maxdiffus / (T)P.cdtv;
     *dt1 = (double)psqrt<T>(dt1_advec * dt1_advec + dt1_diffus * dt1_diffus);
   }
-  *u2_out = (double)u2;
-  *b2_out = (double)b2;
+}
+
+// Whole RHS of one cell: adds it to w[] (already alpha-scaled; slots 0..2 uu,
+// 3 lnrho, [iss], [iax..iax+2]) and returns dt1_max_loc (when DT), u2 and b2.
+template <typename T, bool MAG, bool ENT, bool DT>
+__device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, double* w, int iss,
+                                          int iax, double* dt1, double* u2_out, double* b2_out) {
+  const T rho1 = pexp<T>(-d.lnrho);
+  FieldOut<T> fo{};
+  *b2_out = 0.0;
+  if constexpr (MAG) field_point<T, ENT, DT>(d, rho1, P, w + iax, fo, b2_out);
+  double wf[5];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) wf[v] = w[v];
+  if constexpr (ENT) wf[4] = w[iss];
+  FlowKeep<T> k;
+  if (!ENT && P.polytropic) flow_point_pre<T, ENT, DT, true>(d, rho1, P, wf, k, u2_out);
+  else flow_point_pre<T, ENT, DT, false>(d, rho1, P, wf, k, u2_out);
+  flow_point_post<T, MAG, ENT, DT>(k, fo, P, wf, dt1);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) w[v] = wf[v];
+  if constexpr (ENT) w[iss] = wf[4];
 }
 
 }  // namespace pcmx

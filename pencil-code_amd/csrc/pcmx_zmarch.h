@@ -38,7 +38,7 @@ struct ZPlan {
 };
 
 inline bool zmarch_supported(const pcmx_params& p) {
-  return p.nx % ZM_TX == 0 && p.ny % ZM_TY == 0 && p.nz >= 4;
+  return p.nx % ZM_TX == 0 && p.ny % ZM_TY == 0 && p.nz >= 4 && (long)p.nx * p.ny < (1L << 31);
 }
 
 // Chooses the z-chunk length: the largest divisor of nz that still gives at
@@ -84,6 +84,42 @@ struct ZNeed {
     return v < 3 || (MAG && v >= IA) || (ENT && (v == 3 || v == ISS));
   }
 };
+
+// Files the stencils of variable v at one cell into d (v follows the
+// registration order uu, lnrho, [ss], [aa]; uu components must come in
+// order x, y, z).
+template <typename T, bool MAG, bool ENT>
+__device__ __forceinline__ void file_derivs(Derivs<T>& d, int v, T dx, T dy, T dz, T dxx, T dyy, T dzz, T mxy,
+                                            T mxz, T myz) {
+  using S = Slots<ENT, MAG>;
+  if (v < 3) {
+    d.uij[v][0] = dx;
+    d.uij[v][1] = dy;
+    d.uij[v][2] = dz;
+    add_u_second(d, v, dxx, dyy, dzz, mxy, mxz, myz);
+  } else if (v == 3) {
+    d.glnrho[0] = dx;
+    d.glnrho[1] = dy;
+    d.glnrho[2] = dz;
+    d.del2lnrho = dxx + dyy + dzz;
+  } else if (ENT && v == S::ISS) {
+    d.gss[0] = dx;
+    d.gss[1] = dy;
+    d.gss[2] = dz;
+    d.del2ss = dxx + dyy + dzz;
+  } else if (MAG) {
+    const int k = v - S::IAX;
+    d.aij[k][0] = dx;
+    d.aij[k][1] = dy;
+    d.aij[k][2] = dz;
+    d.d2a[k][0] = dxx;
+    d.d2a[k][1] = dyy;
+    d.d2a[k][2] = dzz;
+    if (k != 2) d.mxa[0][k] = mxy;
+    if (k != 1) d.mxa[1][k] = mxz;
+    if (k != 0) d.mxa[2][k] = myz;
+  }
+}
 
 template <typename T, bool MAG, bool ENT, bool DT>
 __global__ void __launch_bounds__(ZM_NT, 1) k_rhs_zmarch(KArgs a, Phys P, ZPlan zp) {
@@ -245,52 +281,11 @@ __global__ void __launch_bounds__(ZM_NT, 1) k_rhs_zmarch(KArgs a, Phys P, ZPlan 
           myzv = (T)P.facyz * (plus - myz[N::iyz(v)][0]);
         }
         const T val = (sizeof(T) == 4) ? (T)fown[v] : c0;
-        if (v < 3) {
-          d.u[v] = val;
-          d.uij[v][0] = dx;
-          d.uij[v][1] = dy;
-          d.uij[v][2] = dz;
-          d.d2u[v][0] = dxx;
-          d.d2u[v][1] = dyy;
-          d.d2u[v][2] = dzz;
-        } else if (v == 3) {
-          d.lnrho = val;
-          d.glnrho[0] = dx;
-          d.glnrho[1] = dy;
-          d.glnrho[2] = dz;
-          d.d2lnrho[0] = dxx;
-          d.d2lnrho[1] = dyy;
-          d.d2lnrho[2] = dzz;
-        } else if (ENT && v == ISS) {
-          d.ss = val;
-          d.gss[0] = dx;
-          d.gss[1] = dy;
-          d.gss[2] = dz;
-          d.d2ss[0] = dxx;
-          d.d2ss[1] = dyy;
-          d.d2ss[2] = dzz;
-        } else if (MAG) {
-          const int i = v - IA;
-          d.a[i] = val;
-          d.aij[i][0] = dx;
-          d.aij[i][1] = dy;
-          d.aij[i][2] = dz;
-          d.d2a[i][0] = dxx;
-          d.d2a[i][1] = dyy;
-          d.d2a[i][2] = dzz;
-        }
-        if (N::xy(v)) {
-          if (v < 3) d.mxu[0][v] = mxy;
-          else d.mxa[0][v - IA] = mxy;
-        }
-        if (N::xz(v)) {
-          if (v < 3) d.mxu[1][v] = mxzv;
-          else d.mxa[1][v - IA] = mxzv;
-        }
-        if (N::yz(v)) {
-          if (v < 3) d.mxu[2][v] = myzv;
-          else d.mxa[2][v - IA] = myzv;
-        }
+        if (v < 3) d.u[v] = val;
+        else if (v == 3) d.lnrho = val;
+        else if (ENT && v == ISS) d.ss = val;
+        else if (MAG) d.a[v - IA] = val;
+        file_derivs<T, MAG, ENT>(d, v, dx, dy, dz, dxx, dyy, dzz, mxy, mxzv, myzv);
       }
       // carry plane q forward: minus terms of the z-mixed derivatives of the
       // next three planes, and the z-window of the own column

@@ -1,0 +1,420 @@
+// pcmx_zsplit.h -- z-marching fused RHS + RK kernel with the flow and field
+// halves of the RHS on separate wavefronts.
+//
+// Tiling and stencil scheme are those of the first z-marching kernel: a block
+// owns a 32 x 8 column of cells and marches it through a range of z planes;
+// planes of every evolved variable with a 3-cell periodic x/y halo sit in an
+// LDS ring of four (planes q..q+3 while q is produced); z derivatives use the
+// thread's own column (three planes ahead from the ring, three behind in
+// registers); the xz and yz bidiagonal mixed derivatives (deriv.f90:1400-1452)
+// are split as
+//   D_xz f(z) = fac * ( sum_k c_k g_k(z+k) - sum_k c_k g_k(z-k) ),
+//   g_k(z') = f(x+k,z') - f(x-k,z'),  c = (270, -27, 2),
+// the first sum read from the ring, the second carried in registers.
+//
+// What differs:
+//  * With a magnetic field the block has 512 threads: waves 0-3 evaluate the
+//    flow half of the RHS (uu, lnrho, ss: hydro, density, energy, viscosity)
+//    and waves 4-7 the field half (aa: B, J, J x B / rho, induction) for the
+//    same 256 cells.  The field half passes J x B / rho, J^2 and the Alfven
+//    Courant term through LDS.  One block per CU then runs 8 waves (2 per
+//    SIMD) instead of 4, and each thread holds the registers of half the
+//    variables.
+//  * The RK stage is a template parameter, so every memory operation of the
+//    main loop is unconditional and the compiler can count outstanding loads
+//    exactly.
+//  * Software pipeline, one plane deep for every load: while plane q is
+//    computed, plane q+4 (all variables, with halo) and the 2N register of
+//    the own cells of plane q+1 are in flight; each is consumed one phase
+//    after it was issued.
+#pragma once
+
+#include <type_traits>
+
+#include "pcmx_direct.h"
+#include "pcmx_kernels.h"
+#include "pcmx_zmarch.h"
+
+namespace pcmx {
+
+// RK stages of the 2N scheme (timestep.f90:110-161) as the fused kernel sees
+// them.  S1: first substep with a fixed step (f += dt*beta*w, store w);
+// S2: middle substep (reads and stores w); S3: last substep (reads w, does
+// not store it); W: first substep of a Courant-limited step (stores w only,
+// reduces dt1; the update follows once dt is known); DT: reductions only.
+enum Stage { ST_S1 = 0, ST_S2 = 1, ST_S3 = 2, ST_W = 3, ST_DT = 4 };
+
+template <int ST>
+struct StageT {
+  static constexpr bool READW = ST == ST_S2 || ST == ST_S3;
+  static constexpr bool WRITEW = ST == ST_S1 || ST == ST_S2 || ST == ST_W;
+  static constexpr bool UPDATE = ST == ST_S1 || ST == ST_S2 || ST == ST_S3;
+  static constexpr bool DT = ST == ST_W || ST == ST_DT;
+  static constexpr bool DIAG = ST == ST_S1 || ST == ST_W || ST == ST_DT;
+};
+
+template <bool MAG>
+constexpr int zs_threads() {
+  return MAG ? 2 * ZM_NT : ZM_NT;
+}
+
+template <typename T, bool MAG, bool ENT, bool POLY, int ST>
+__global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
+  using S = Slots<ENT, MAG>;
+  using N = ZNeed<MAG, ENT>;
+  using G = StageT<ST>;
+  constexpr bool DT = G::DT;
+  constexpr int NV = S::NV, IA = S::IAX, ISS = S::ISS;
+  constexpr int NTOT = zs_threads<MAG>();
+  constexpr int SLOT = NV * ZM_PLANE;
+  constexpr int NF = ENT ? 5 : 4;              // flow variables: uu, lnrho, [ss]
+  constexpr int NGV = NF;                      // largest group (the flow half)
+  constexpr int NXC = 5;                       // exchange slots: jxbr[3], j2, advec_va2
+  constexpr bool MIXED = sizeof(T) == 4;
+  __shared__ T ring[ZM_RING * SLOT];
+  __shared__ T xch[MAG ? NXC * ZM_NT : 1];
+
+  const int tid = threadIdx.x;
+  const int grp = MAG ? tid / ZM_NT : 0;  // wave-uniform
+  const int t = tid % ZM_NT;
+  const int tx = t % ZM_TX, ty = t / ZM_TX;
+  // XCD-aware work order: consecutive work items (neighbouring tiles that
+  // share halo rims) go to the same XCD, whose L2 then serves the rims.
+  const long b = blockIdx.x;
+  const long wid = zp.xcd_remap ? (b % 8) * (zp.nblocks / 8) + b / 8 : b;
+  const int txt = (int)(wid % zp.ntx);
+  const long rest = wid / zp.ntx;
+  const int tyt = (int)(rest % zp.nty);
+  const int chunk = (int)(rest / zp.nty);
+  const int x0 = txt * ZM_TX, y0 = tyt * ZM_TY, z0 = chunk * zp.lz;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long nxy = (long)nx * ny, nxyz = a.nxyz;
+  const int zlast = z0 + zp.lz - 1;
+
+  auto zoff = [&](int p) -> long {
+    int zz = p % nz;
+    zz += (zz < 0) ? nz : 0;
+    return (long)zz * nxy;
+  };
+  auto slot_of = [&](int p) -> int { return (p - z0 + 3) & 3; };
+  // offset inside a z plane of element ee of the tile's haloed plane
+  auto plane_off = [&](int ee) -> unsigned {
+    const int row = ee / ZM_PX, col = ee % ZM_PX;
+    int gx = x0 - 3 + col, gy = y0 - 3 + row;
+    gx += (gx < 0) ? nx : 0;
+    gx -= (gx >= nx) ? nx : 0;
+    gy += (gy < 0) ? ny : 0;
+    gy -= (gy >= ny) ? ny : 0;
+    return (unsigned)(gy * nx + gx);
+  };
+
+  // Plane staging.  Each variable's haloed plane (ZM_PLANE values) is
+  // covered by K full slots, element tid + k*NTOT of every variable (so a
+  // load is a wave-uniform base, variable and plane, plus a 32-bit lane
+  // offset), and one remainder slot: the last R elements of every variable,
+  // NV*R of them, one each for threads tid < NV*R.
+  constexpr int K = ZM_PLANE / NTOT;
+  constexpr int R = ZM_PLANE - K * NTOT;
+  static_assert(NV * R <= NTOT, "remainder slot must fit in one pass");
+  unsigned poff[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) poff[k] = plane_off(tid + k * NTOT);
+  const bool has_rem = tid < NV * R;
+  const int rv = has_rem ? tid / R : 0;
+  const int re = K * NTOT + (has_rem ? tid % R : 0);
+  const unsigned roff = plane_off(re);
+  const int rl = rv * ZM_PLANE + re;
+  const double rmu = MIXED ? a.means[rv] : 0.0;
+
+  double pre[NV * K], prem = 0.0;
+  auto load_plane = [&](int p) {
+    const double* src = a.fin + zoff(p);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* sv = src + v * nxyz;
+#pragma unroll
+      for (int k = 0; k < K; ++k) pre[v * K + k] = sv[poff[k]];
+    }
+    if (has_rem) prem = src[rv * nxyz + roff];
+  };
+  auto store_plane = [&](int p) {
+    T* dst = ring + slot_of(p) * SLOT;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double mu = MIXED ? a.means[v] : 0.0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double val = pre[v * K + k];
+        dst[v * ZM_PLANE + tid + k * NTOT] = MIXED ? (T)(val - mu) : (T)val;
+      }
+    }
+    if (has_rem) dst[rl] = MIXED ? (T)(prem - rmu) : (T)prem;
+  };
+
+  // the variables of this thread's group: flow vi -> 0,1,2,3,[ISS]; field vi -> IA+vi
+  auto var_of = [&](int g, int i) -> int { return g == 0 ? (i < 4 ? i : ISS) : IA + i; };
+  const int x = x0 + tx, y = y0 + ty;
+  const unsigned own = (unsigned)(y * nx + x);  // offset inside a z plane
+  const int c = (ty + 3) * ZM_PX + (tx + 3);
+  // start of variable v's z plane p in an array of the state layout (uniform)
+  auto vplane = [&](auto* base, int v, int p) { return base + ((long)v * nxyz + (long)p * nxy); };
+
+  // own-cell prefetch of the next output plane: 2N register (READW) and, in
+  // the mixed form, the FP64 state values (the ring holds FP32 offsets)
+  constexpr int NOWNF = MIXED ? (MAG ? 4 : 0) : 0;  // field half also needs u, lnrho in FP64
+  double wnx[NGV], fnx[NGV + NOWNF];
+  auto prefetch_own = [&](int g, int p) {
+    p = p < z0 ? z0 : (p > zlast ? zlast : p);  // stays inside this block's chunk
+    if constexpr (G::READW) {
+#pragma unroll
+      for (int i = 0; i < NGV; ++i)
+        if (g == 0 ? i < NF : i < 3) wnx[i] = vplane(a.w, var_of(g, i), p)[own];
+    }
+    if constexpr (MIXED) {
+#pragma unroll
+      for (int i = 0; i < NGV; ++i)
+        if (g == 0 ? i < NF : i < 3) fnx[i] = vplane(a.fin, var_of(g, i), p)[own];
+      if constexpr (MAG) {
+        if (g == 1) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) fnx[NGV + i] = vplane(a.fin, i, p)[own];
+        }
+      }
+    }
+  };
+
+  // ---- prime the ring with planes z0-3 .. z0, start plane z0+1 ----
+  for (int p = z0 - 3; p <= z0; ++p) {
+    load_plane(p);
+    store_plane(p);
+  }
+  load_plane(z0 + 1);
+  prefetch_own(grp, z0);
+  __syncthreads();
+
+  // carried state: z window of the own column, minus parts of the z-mixed
+  // derivatives (group-local slots: xz of uu_x|aa_x -> 0, uu_z|aa_z -> 1;
+  // yz of uu_y|aa_y -> 0, uu_z|aa_z -> 1)
+  T back[NGV][3], cxz[2][3], cyz[2][3];
+#pragma unroll
+  for (int i = 0; i < NGV; ++i) back[i][0] = back[i][1] = back[i][2] = (T)0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    cxz[i][0] = cxz[i][1] = cxz[i][2] = (T)0;
+    cyz[i][0] = cyz[i][1] = cyz[i][2] = (T)0;
+  }
+  double red[NRED];
+#pragma unroll
+  for (int q = 0; q < NRED; ++q) red[q] = 0.0;
+
+  const T fx1 = (T)P.fac1[0], fy1 = (T)P.fac1[1], fz1 = (T)P.fac1[2];
+  const T fx2 = (T)P.fac2[0], fy2 = (T)P.fac2[1], fz2 = (T)P.fac2[2];
+  const T C0 = (T)PCMX_D2C0, C1 = (T)PCMX_D2C1, C2 = (T)PCMX_D2C2, C3 = (T)PCMX_D2C3;
+  const double dtb = a.d_dt ? a.d_dt[0] * a.beta : a.dtb;
+
+  // stencils of variable v (group-local index i) at plane q; carries updated
+  // with plane q.  own64 receives the FP64 value of the own cell (FP64 form).
+  auto derive = [&](auto emit_c, int v, int i, Derivs<T>& d, const T* L0, const T* L1, const T* L2,
+                    const T* L3) {
+    constexpr bool EMIT = decltype(emit_c)::value;
+    const T* p0 = L0 + v * ZM_PLANE;
+    const T c0 = p0[0];
+    const T xp1 = p0[1], xm1 = p0[-1], xp2 = p0[2], xm2 = p0[-2], xp3 = p0[3], xm3 = p0[-3];
+    const T yp1 = p0[ZM_PX], ym1 = p0[-ZM_PX], yp2 = p0[2 * ZM_PX], ym2 = p0[-2 * ZM_PX];
+    const T yp3 = p0[3 * ZM_PX], ym3 = p0[-3 * ZM_PX];
+    if constexpr (EMIT) {
+      const T* p1 = L1 + v * ZM_PLANE;
+      const T* p2 = L2 + v * ZM_PLANE;
+      const T* p3 = L3 + v * ZM_PLANE;
+      const T zp1 = p1[0], zp2 = p2[0], zp3 = p3[0];
+      const T zm1 = back[i][0], zm2 = back[i][1], zm3 = back[i][2];
+      // der_main (deriv.f90:124-162)
+      const T dx = fx1 * ((T)45.0 * (xp1 - xm1) - (T)9.0 * (xp2 - xm2) + (xp3 - xm3));
+      const T dy = fy1 * ((T)45.0 * (yp1 - ym1) - (T)9.0 * (yp2 - ym2) + (yp3 - ym3));
+      const T dz = fz1 * ((T)45.0 * (zp1 - zm1) - (T)9.0 * (zp2 - zm2) + (zp3 - zm3));
+      T dxx = 0, dyy = 0, dzz = 0;
+      if (N::d2(v)) {  // der2_main (deriv.f90:497-550)
+        dxx = fx2 * (C0 * c0 + C1 * (xp1 + xm1) + C2 * (xp2 + xm2) + C3 * (xp3 + xm3));
+        dyy = fy2 * (C0 * c0 + C1 * (yp1 + ym1) + C2 * (yp2 + ym2) + C3 * (yp3 + ym3));
+        dzz = fz2 * (C0 * c0 + C1 * (zp1 + zm1) + C2 * (zp2 + zm2) + C3 * (zp3 + zm3));
+      }
+      T mxy = 0, mxzv = 0, myzv = 0;
+      if (N::xy(v)) {  // bidiagonal xy, operand order of deriv.f90:1410-1418
+        const T b1 = p0[1 + ZM_PX] - p0[-1 + ZM_PX] + p0[-1 - ZM_PX] - p0[1 - ZM_PX];
+        const T b2 = p0[2 + 2 * ZM_PX] - p0[-2 + 2 * ZM_PX] + p0[-2 - 2 * ZM_PX] - p0[2 - 2 * ZM_PX];
+        const T b3 = p0[3 + 3 * ZM_PX] - p0[-3 + 3 * ZM_PX] + p0[-3 - 3 * ZM_PX] - p0[3 - 3 * ZM_PX];
+        mxy = (T)P.facxy * ((T)270.0 * b1 - (T)27.0 * b2 + (T)2.0 * b3);
+      }
+      if (N::xz(v)) {
+        const T plus = (T)270.0 * (p1[1] - p1[-1]) - (T)27.0 * (p2[2] - p2[-2]) + (T)2.0 * (p3[3] - p3[-3]);
+        mxzv = (T)P.facxz * (plus - cxz[N::ixz(v) & 1][0]);
+      }
+      if (N::yz(v)) {
+        const T plus = (T)270.0 * (p1[ZM_PX] - p1[-ZM_PX]) - (T)27.0 * (p2[2 * ZM_PX] - p2[-2 * ZM_PX]) +
+                       (T)2.0 * (p3[3 * ZM_PX] - p3[-3 * ZM_PX]);
+        myzv = (T)P.facyz * (plus - cyz[N::iyz(v) & 1][0]);
+      }
+      file_derivs<T, MAG, ENT>(d, v, dx, dy, dz, dxx, dyy, dzz, mxy, mxzv, myzv);
+    }
+    // carry plane q forward (minus parts of the z-mixed derivatives of the
+    // next three planes, and the own column's z window)
+    if (N::xz(v)) {
+      T* m = cxz[N::ixz(v) & 1];
+      m[0] = m[1] + (T)270.0 * (xp1 - xm1);
+      m[1] = m[2] - (T)27.0 * (xp2 - xm2);
+      m[2] = (T)2.0 * (xp3 - xm3);
+    }
+    if (N::yz(v)) {
+      T* m = cyz[N::iyz(v) & 1];
+      m[0] = m[1] + (T)270.0 * (yp1 - ym1);
+      m[1] = m[2] - (T)27.0 * (yp2 - ym2);
+      m[2] = (T)2.0 * (yp3 - ym3);
+    }
+    back[i][2] = back[i][1];
+    back[i][1] = back[i][0];
+    back[i][0] = c0;
+    return c0;
+  };
+
+  // values kept by the flow half between its two phases
+  double wf[5], ff[5];
+  FlowKeep<T> keep;
+  double u2c = 0.0;
+
+  auto iteration = [&](auto emit_c, int q) {
+    constexpr bool EMIT = decltype(emit_c)::value;
+    const T* L0 = ring + slot_of(q) * SLOT + c;
+    const T* L1 = ring + slot_of(q + 1) * SLOT + c;
+    const T* L2 = ring + slot_of(q + 2) * SLOT + c;
+    const T* L3 = ring + slot_of(q + 3) * SLOT + c;
+    // ---------------- phase 1 ----------------
+    double wcur[NGV], fcur[NGV + NOWNF];
+#pragma unroll
+    for (int i = 0; i < NGV; ++i) wcur[i] = wnx[i];
+#pragma unroll
+    for (int i = 0; i < NGV + NOWNF; ++i) fcur[i] = fnx[i];
+    if (grp == 0) {
+      Derivs<T> d;
+#pragma unroll
+      for (int i = 0; i < NF; ++i) {
+        const int v = var_of(0, i);
+        const T c0 = derive(emit_c, v, i, d, L0, L1, L2, L3);
+        if constexpr (EMIT) {
+          ff[i] = MIXED ? fcur[i] : (double)c0;
+          const T val = (T)ff[i];
+          if (v < 3) d.u[v] = val;
+          else if (v == 3) d.lnrho = val;
+          else d.ss = val;
+        }
+      }
+      if constexpr (EMIT) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) wf[i] = G::READW ? a.alpha * wcur[i] : 0.0;
+        const T rho1 = pexp<T>(-d.lnrho);
+        flow_point_pre<T, ENT, DT, POLY>(d, rho1, P, wf, keep, &u2c);
+      }
+    } else if constexpr (MAG) {
+      Derivs<T> d;
+      double fa[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int v = IA + i;
+        const T c0 = derive(emit_c, v, i, d, L0, L1, L2, L3);
+        if constexpr (EMIT) {
+          fa[i] = MIXED ? fcur[i] : (double)c0;
+          d.a[i] = (T)fa[i];
+        }
+      }
+      if constexpr (EMIT) {
+        T lnrho;
+        if constexpr (MIXED) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) d.u[j] = (T)fcur[NGV + j];
+          lnrho = (T)fcur[NGV + 3];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) d.u[j] = L0[j * ZM_PLANE];
+          lnrho = L0[3 * ZM_PLANE];
+        }
+        const T rho1 = pexp<T>(-lnrho);
+        double wa[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) wa[i] = G::READW ? a.alpha * wcur[i] : 0.0;
+        FieldOut<T> fo;
+        double b2 = 0.0;
+        field_point<T, ENT, DT>(d, rho1, P, wa, fo, &b2);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) xch[j * ZM_NT + t] = fo.jxbr[j];
+        if constexpr (ENT) xch[3 * ZM_NT + t] = fo.j2;
+        if constexpr (DT) xch[4 * ZM_NT + t] = fo.advec_va2;
+        if constexpr (G::UPDATE) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) vplane(a.fout, IA + i, q)[own] = fa[i] + dtb * wa[i];
+        }
+        if constexpr (G::WRITEW) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) vplane(a.w, IA + i, q)[own] = wa[i];
+        }
+        if constexpr (G::DIAG) {
+          red[2] += b2;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) red[4 + IA + i] += fa[i];
+        }
+      }
+    }
+    prefetch_own(grp, q + 1);
+    __syncthreads();
+    // ---------------- phase 2 ----------------
+    if constexpr (EMIT) {
+      if (grp == 0) {
+        FieldOut<T> fo{};
+        if constexpr (MAG) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) fo.jxbr[j] = xch[j * ZM_NT + t];
+          if constexpr (ENT) fo.j2 = xch[3 * ZM_NT + t];
+          if constexpr (DT) fo.advec_va2 = xch[4 * ZM_NT + t];
+        }
+        double dt1 = 0.0;
+        flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
+        if constexpr (G::UPDATE) {
+#pragma unroll
+          for (int i = 0; i < NF; ++i) vplane(a.fout, var_of(0, i), q)[own] = ff[i] + dtb * wf[i];
+        }
+        if constexpr (G::WRITEW) {
+#pragma unroll
+          for (int i = 0; i < NF; ++i) vplane(a.w, var_of(0, i), q)[own] = wf[i];
+        }
+        if constexpr (G::DIAG) {
+          red[0] += u2c;
+          red[1] = fmax(red[1], u2c);
+          if constexpr (DT) red[3] = fmax(red[3], dt1);
+#pragma unroll
+          for (int i = 0; i < NF; ++i) red[4 + var_of(0, i)] += ff[i];
+        }
+      }
+    }
+    // refill: plane q+4 replaces plane q in the ring; start plane q+5
+    store_plane(q + 4);
+    if (q + 5 <= zlast + 3) load_plane(q + 5);
+    __syncthreads();
+  };
+
+  for (int q = z0 - 3; q < z0; ++q) iteration(std::false_type{}, q);
+  for (int q = z0; q <= zlast; ++q) iteration(std::true_type{}, q);
+
+  if constexpr (G::DIAG) {
+    if (a.partials) block_reduce_store<NTOT>(red, a.partials, (long)blockIdx.x);
+  }
+}
+
+template <typename T, bool MAG, bool ENT, int ST>
+void launch_zsplit(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
+  if (!ENT && P.polytropic)
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST>), dim3((unsigned)zp.nblocks), dim3(zs_threads<MAG>()),
+                       0, s, a, P, zp);
+  else
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST>), dim3((unsigned)zp.nblocks), dim3(zs_threads<MAG>()),
+                       0, s, a, P, zp);
+}
+
+}  // namespace pcmx

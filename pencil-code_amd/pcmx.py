@@ -30,7 +30,7 @@ N_GPU_REDUCED_VARS = 20
 
 PREC_FP64 = 0
 PREC_MIXED = 1
-KERNEL_AUTO, KERNEL_DIRECT, KERNEL_ZMARCH = 0, 1, 2
+KERNEL_AUTO, KERNEL_DIRECT, KERNEL_ZMARCH, KERNEL_ZSPLIT = 0, 1, 2, 3
 
 
 class PcmxError(RuntimeError):
@@ -58,7 +58,8 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, c
     (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
     registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
     kernel None takes PCMX_KERNEL from the environment (0 auto, 1 direct,
-    2 z-marching), default auto."""
+    2 z-marching, 3 z-marching with the flow and field halves on separate
+    wavefronts), default auto."""
     if kernel is None:
         kernel = int(os.environ.get("PCMX_KERNEL", KERNEL_AUTO))
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
