@@ -356,16 +356,13 @@ int pcmx_load_farray(const double* f, int mfarray) {
   a.mode = MODE_DT;
   a.partials = e.partials;
   // the mixed form needs means before its first use: compute them with the
-  // FP64 direct kernel's reduction path
-  const int saved_kernel = e.kernel;
+  // FP64 variant of the engine's kernel
   const int saved_prec = e.par.precision;
-  e.kernel = 1;
   e.par.precision = PCMX_PREC_FP64;
   rc = launch_rhs(e, a, true);
-  e.kernel = saved_kernel;
   e.par.precision = saved_prec;
   if (rc) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, e.stream, e.partials, direct_blocks(e.par), e.results,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, e.stream, e.partials, nblocks(e), e.results,
                      (double)e.nxyz, e.nvar, 0, 1);
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(e.stream));

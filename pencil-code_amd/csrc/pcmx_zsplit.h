@@ -27,6 +27,9 @@
 //    computed, plane q+4 (all variables, with halo) and the 2N register of
 //    the own cells of plane q+1 are in flight; each is consumed one phase
 //    after it was issued.
+//  * Data touched once (the own-cell 2N register loads, every state and 2N
+//    register store) use non-temporal accesses, so that the L2 keeps the
+//    halo rims that neighbouring tiles read again.
 #pragma once
 
 #include <type_traits>
@@ -163,22 +166,20 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   // the mixed form, the FP64 state values (the ring holds FP32 offsets)
   constexpr int NOWNF = MIXED ? (MAG ? 4 : 0) : 0;  // field half also needs u, lnrho in FP64
   double wnx[NGV], fnx[NGV + NOWNF];
-  auto prefetch_own = [&](int g, int p) {
+  auto prefetch_own = [&](auto g_c, int p) {
+    constexpr int g = decltype(g_c)::value;
+    constexpr int ngv = g == 0 ? NF : 3;
     p = p < z0 ? z0 : (p > zlast ? zlast : p);  // stays inside this block's chunk
     if constexpr (G::READW) {
 #pragma unroll
-      for (int i = 0; i < NGV; ++i)
-        if (g == 0 ? i < NF : i < 3) wnx[i] = vplane(a.w, var_of(g, i), p)[own];
+      for (int i = 0; i < ngv; ++i) wnx[i] = __builtin_nontemporal_load(&vplane(a.w, var_of(g, i), p)[own]);
     }
     if constexpr (MIXED) {
 #pragma unroll
-      for (int i = 0; i < NGV; ++i)
-        if (g == 0 ? i < NF : i < 3) fnx[i] = vplane(a.fin, var_of(g, i), p)[own];
-      if constexpr (MAG) {
-        if (g == 1) {
+      for (int i = 0; i < ngv; ++i) fnx[i] = vplane(a.fin, var_of(g, i), p)[own];
+      if constexpr (MAG && g == 1) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) fnx[NGV + i] = vplane(a.fin, i, p)[own];
-        }
+        for (int i = 0; i < 4; ++i) fnx[NGV + i] = vplane(a.fin, i, p)[own];
       }
     }
   };
@@ -188,9 +189,6 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     load_plane(p);
     store_plane(p);
   }
-  load_plane(z0 + 1);
-  prefetch_own(grp, z0);
-  __syncthreads();
 
   // carried state: z window of the own column, minus parts of the z-mixed
   // derivatives (group-local slots: xz of uu_x|aa_x -> 0, uu_z|aa_z -> 1;
@@ -281,8 +279,9 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   FlowKeep<T> keep;
   double u2c = 0.0;
 
-  auto iteration = [&](auto emit_c, int q) {
+  auto iteration = [&](auto emit_c, auto g_c, int q) {
     constexpr bool EMIT = decltype(emit_c)::value;
+    constexpr int GRP = decltype(g_c)::value;
     const T* L0 = ring + slot_of(q) * SLOT + c;
     const T* L1 = ring + slot_of(q + 1) * SLOT + c;
     const T* L2 = ring + slot_of(q + 2) * SLOT + c;
@@ -293,7 +292,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     for (int i = 0; i < NGV; ++i) wcur[i] = wnx[i];
 #pragma unroll
     for (int i = 0; i < NGV + NOWNF; ++i) fcur[i] = fnx[i];
-    if (grp == 0) {
+    if constexpr (GRP == 0) {
       Derivs<T> d;
 #pragma unroll
       for (int i = 0; i < NF; ++i) {
@@ -349,11 +348,11 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         if constexpr (DT) xch[4 * ZM_NT + t] = fo.advec_va2;
         if constexpr (G::UPDATE) {
 #pragma unroll
-          for (int i = 0; i < 3; ++i) vplane(a.fout, IA + i, q)[own] = fa[i] + dtb * wa[i];
+          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(fa[i] + dtb * wa[i], &vplane(a.fout, IA + i, q)[own]);
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
-          for (int i = 0; i < 3; ++i) vplane(a.w, IA + i, q)[own] = wa[i];
+          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(wa[i], &vplane(a.w, IA + i, q)[own]);
         }
         if constexpr (G::DIAG) {
           red[2] += b2;
@@ -362,11 +361,11 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         }
       }
     }
-    prefetch_own(grp, q + 1);
+    prefetch_own(g_c, q + 1);
     __syncthreads();
     // ---------------- phase 2 ----------------
     if constexpr (EMIT) {
-      if (grp == 0) {
+      if constexpr (GRP == 0) {
         FieldOut<T> fo{};
         if constexpr (MAG) {
 #pragma unroll
@@ -378,11 +377,11 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
         if constexpr (G::UPDATE) {
 #pragma unroll
-          for (int i = 0; i < NF; ++i) vplane(a.fout, var_of(0, i), q)[own] = ff[i] + dtb * wf[i];
+          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(ff[i] + dtb * wf[i], &vplane(a.fout, var_of(0, i), q)[own]);
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
-          for (int i = 0; i < NF; ++i) vplane(a.w, var_of(0, i), q)[own] = wf[i];
+          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(wf[i], &vplane(a.w, var_of(0, i), q)[own]);
         }
         if constexpr (G::DIAG) {
           red[0] += u2c;
@@ -393,14 +392,33 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         }
       }
     }
-    // refill: plane q+4 replaces plane q in the ring; start plane q+5
+    // refill: plane q+4 replaces plane q in the ring; start plane q+5 (past
+    // the last plane the block needs, the last one is fetched again, an L2
+    // hit, so that every iteration issues the same memory operations)
     store_plane(q + 4);
-    if (q + 5 <= zlast + 3) load_plane(q + 5);
+    load_plane(q + 5 <= zlast + 3 ? q + 5 : zlast + 3);
     __syncthreads();
   };
 
-  for (int q = z0 - 3; q < z0; ++q) iteration(std::false_type{}, q);
-  for (int q = z0; q <= zlast; ++q) iteration(std::true_type{}, q);
+  // Each wave group runs its own copy of the marching loop: inside a loop
+  // every wave then issues one fixed sequence of memory operations, which
+  // lets the compiler count outstanding loads exactly (a loop shared by both
+  // groups, with group branches around unequal numbers of loads and stores,
+  // ends in conservative full waits).  Both copies pass the same barriers.
+  auto run = [&](auto g_c) {
+    // same issue order as the loop body: own-cell loads, then the plane
+    prefetch_own(g_c, z0);
+    load_plane(z0 + 1);
+    __syncthreads();
+    for (int q = z0 - 3; q < z0; ++q) iteration(std::false_type{}, g_c, q);
+    for (int q = z0; q <= zlast; ++q) iteration(std::true_type{}, g_c, q);
+  };
+  if constexpr (MAG) {
+    if (grp == 1) run(std::integral_constant<int, 1>{});
+    else run(std::integral_constant<int, 0>{});
+  } else {
+    run(std::integral_constant<int, 0>{});
+  }
 
   if constexpr (G::DIAG) {
     if (a.partials) block_reduce_store<NTOT>(red, a.partials, (long)blockIdx.x);
