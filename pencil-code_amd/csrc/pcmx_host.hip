@@ -58,6 +58,7 @@ struct Engine {
   bool mag = false, ent = false;
   double* f[2] = {nullptr, nullptr};
   double* w = nullptr;
+  double* w2 = nullptr;       // second 2N register of the deferred Courant step (kernel 3)
   int cur = 0;
   double* partials = nullptr;
   long partials_cap = 0;
@@ -71,6 +72,9 @@ struct Engine {
   double timing_ms = 0.0;
   long timing_n = 0;
   double last_dt = 0.0;
+  // kernel 3 defers the first substep's update to the second substep: after
+  // substep 1, f[cur] still holds f0 and f1 = f0 + dt*beta1*w1 is implicit
+  bool pending_f1 = false;
 };
 
 Engine* g = nullptr;
@@ -141,22 +145,15 @@ void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
 }
 long direct_blocks(const pcmx_params& p) { return (long)((p.nx + 63) / 64) * ((p.ny + 3) / 4) * p.nz; }
 
-// RK stage of a launch of the split z-marching kernel
-int stage_of(const KArgs& a) {
-  if (a.mode == MODE_DT) return ST_DT;
-  if (a.mode == MODE_W) return ST_W;
-  if (a.first) return ST_S1;
-  return a.write_w ? ST_S2 : ST_S3;
-}
-
 template <typename T, bool MAG, bool ENT>
 void launch_t(const Engine& e, const KArgs& a, bool dt) {
   if (e.kernel == 3) {
-    switch (stage_of(a)) {
+    switch (a.stage) {
       case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1>(a, e.phys, e.zplan, e.stream); break;
       case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2>(a, e.phys, e.zplan, e.stream); break;
       case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3>(a, e.phys, e.zplan, e.stream); break;
       case ST_W: launch_zsplit<T, MAG, ENT, ST_W>(a, e.phys, e.zplan, e.stream); break;
+      case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D>(a, e.phys, e.zplan, e.stream); break;
       default: launch_zsplit<T, MAG, ENT, ST_DT>(a, e.phys, e.zplan, e.stream); break;
     }
   } else if (e.kernel == 2) {
@@ -198,6 +195,7 @@ KArgs base_args(Engine& e) {
   a.iax = e.par.iax;
   a.nxyz = e.nxyz;
   a.w = e.w;
+  a.wout = e.w;
   a.means = e.results + R_MEAN0;
   return a;
 }
@@ -221,11 +219,13 @@ int substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullpt
   a.beta = BETA_TS[s];
   a.dtb = dt_host * BETA_TS[s];  // dt_beta_ts = dt*beta_ts (timestep.f90:150,215)
   a.write_w = (s < 2);
+  a.stage = (s == 0) ? ST_S1 : (s == 1 ? ST_S2 : ST_S3);
   const bool ldt = e.par.ldt != 0;
   if (ldt && s == 0) {
     // dt follows from this substep's RHS (equ.f90:408) before f is updated:
     // store w1, reduce dt1, then f1 = f0 + dt*beta1*w1.
     a.mode = MODE_W;
+    a.stage = ST_W;
     a.partials = e.partials;
     if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
     int rc = launch_rhs(e, a, true);
@@ -251,11 +251,73 @@ int substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullpt
   return PCMX_OK;
 }
 
+// One RK substep of a Courant-limited step with the deferred first-substep
+// update (kernel 3): substep 1 stores only w1 = RHS(f0) and reduces dt1 (dt
+// is then on the device before substep 2 needs it); substep 2 forms
+// f1 = f0 + dt*beta1*w1 while staging and stores f2 and w2 (into the second
+// 2N buffer: neighbouring blocks still read w1 in their halos); substep 3
+// stores f3.  The buffer holding f3 is the one that held f0.
+int substep_deferred(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0, hipEvent_t ev1) {
+  KArgs a = base_args(e);
+  const bool ldt = e.par.ldt != 0;
+  a.alpha = ALPHA_TS[s];
+  a.beta = BETA_TS[s];
+  a.dtb = dt_host * BETA_TS[s];  // dt_beta_ts = dt*beta_ts (timestep.f90:150,215)
+  a.beta1 = BETA_TS[0];
+  a.dtb1 = dt_host * BETA_TS[0];
+  a.d_dt = ldt ? e.results + R_DT : nullptr;
+  a.first = (s == 0);
+  a.write_w = (s < 2);
+  a.fin = e.f[e.cur];
+  a.fout = (s == 0) ? nullptr : e.f[1 - e.cur];
+  if (s == 0) {
+    a.stage = ST_W;
+    a.mode = MODE_W;
+    a.partials = e.partials;
+  } else if (s == 1) {
+    a.stage = ST_S2D;
+    a.mode = MODE_UPDATE;
+    a.wout = e.w2;
+  } else {
+    a.stage = ST_S3;
+    a.mode = MODE_UPDATE;
+    a.w = e.w2;
+  }
+  if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
+  int rc = launch_rhs(e, a, s == 0 && ldt);
+  if (rc) return rc;
+  if (ev1) HIPCHECK(hipEventRecord(ev1, e.stream));
+  (void)diag;
+  if (s == 0) {
+    if ((rc = finalize_red(e, true, true))) return rc;
+    e.pending_f1 = true;
+  } else {
+    e.pending_f1 = false;
+    e.cur = 1 - e.cur;
+  }
+  return PCMX_OK;
+}
+
+int run_substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  if (e.kernel == 3 && e.par.ldt) return substep_deferred(e, s, dt_host, diag, ev0, ev1);
+  return substep(e, s, dt_host, diag, ev0, ev1);
+}
+
+// The state as the reference would hold it: after substep 1 of kernel 3,
+// f1 = f0 + dt*beta1*w1 is formed into the spare buffer.  Returns the
+// buffer to read.
+double* current_state(Engine& e) {
+  if (!e.pending_f1) return e.f[e.cur];
+  hipLaunchKernelGGL(k_axpy_dt, dim3(2048), dim3(256), 0, e.stream, e.f[e.cur], e.f[1 - e.cur], e.w,
+                     e.results + R_DT, BETA_TS[0], e.nxyz * e.nvar);
+  return e.f[1 - e.cur];
+}
+
 int require() { return g ? PCMX_OK : fail(PCMX_ERR_STATE, "pcmx_initialize has not been called"); }
 
 void free_engine(Engine* e) {
   if (!e) return;
-  for (double* p : {e->f[0], e->f[1], e->w, e->partials, e->results, e->staging})
+  for (double* p : {e->f[0], e->f[1], e->w, e->w2, e->partials, e->results, e->staging})
     if (p) (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -306,7 +368,8 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   e->partials_cap = std::max(direct_blocks(*par), e->kernel >= 2 ? e->zplan.nblocks : 0L);
   if ((rc = alloc(&e->f[0], bytes)) || (rc = alloc(&e->f[1], bytes)) || (rc = alloc(&e->w, bytes)) ||
       (rc = alloc(&e->partials, sizeof(double) * NRED * e->partials_cap)) ||
-      (rc = alloc(&e->results, sizeof(double) * R_COUNT))) {
+      (rc = alloc(&e->results, sizeof(double) * R_COUNT)) ||
+      (e->kernel == 3 && par->ldt && (rc = alloc(&e->w2, bytes)))) {
     free_engine(e);
     return rc;
   }
@@ -349,11 +412,13 @@ int pcmx_load_farray(const double* f, int mfarray) {
     HIPCHECK(hipGetLastError());
   }
   // box means for the mixed form, diagnostics and dt1 of the loaded state
+  e.pending_f1 = false;
   KArgs a = base_args(e);
   a.fin = e.f[e.cur];
   a.fout = nullptr;
   a.first = 1;
   a.mode = MODE_DT;
+  a.stage = ST_DT;
   a.partials = e.partials;
   // the mixed form needs means before its first use: compute them with the
   // FP64 variant of the engine's kernel
@@ -377,8 +442,9 @@ int pcmx_copy_farray(double* f, int mfarray) {
   if (mfarray < e.nvar) return fail(PCMX_ERR_ARG, "mfarray=%d < mvar=%d", mfarray, e.nvar);
   const long mw = mw_of(e.par);
   if (!e.staging) HIPCHECK(hipMalloc((void**)&e.staging, sizeof(double) * mw));
+  const double* state = current_state(e);
   for (int v = 0; v < e.nvar; ++v) {
-    hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, e.f[e.cur] + (long)v * e.nxyz,
+    hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, state + (long)v * e.nxyz,
                        e.staging, e.par.nx, e.par.ny, e.par.nz);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(f + (long)v * mw, e.staging, sizeof(double) * mw, hipMemcpyDeviceToHost, e.stream));
@@ -407,7 +473,7 @@ int pcmx_rhs(int isubstep, double t, double* dt) {
     double r[1] = {*dt};
     HIPCHECK(hipMemcpyAsync(e.results + R_DT, r, sizeof(double), hipMemcpyHostToDevice, e.stream));
   }
-  if ((rc = substep(e, s, *dt, true))) return rc;
+  if ((rc = run_substep(e, s, *dt, true))) return rc;
   if (e.par.ldt && s == 0) {
     double r;
     HIPCHECK(hipMemcpyAsync(&r, e.results + R_DT, sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -429,9 +495,10 @@ int pcmx_prepare_for_first_substep(double t, double* dt1) {
   if (rc) return rc;
   Engine& e = *g;
   KArgs a = base_args(e);
-  a.fin = e.f[e.cur];
+  a.fin = current_state(e);
   a.first = 1;
   a.mode = MODE_DT;
+  a.stage = ST_DT;
   a.partials = e.partials;
   if ((rc = launch_rhs(e, a, true))) return rc;
   if ((rc = finalize_red(e, false, false))) return rc;
@@ -582,7 +649,7 @@ int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag) {
     for (int s = 0; s < 3; ++s) {
       hipEvent_t e0 = e.timing ? evs[6L * it + 2 * s] : nullptr;
       hipEvent_t e1 = e.timing ? evs[6L * it + 2 * s + 1] : nullptr;
-      if ((rc = substep(e, s, *dt, ldiag != 0, e0, e1))) return rc;
+      if ((rc = run_substep(e, s, *dt, ldiag != 0, e0, e1))) return rc;
       if (ldt && s == 0) {
         // t advances with the step of this substep's Courant condition
         double r;

@@ -24,6 +24,8 @@ struct KArgs {
   const double* fin;    // state f at the start of the substep (interior, [v][z][y][x])
   double* fout;         // state after the substep
   double* w;            // 2N register (df), read when !first, written when write_w
+  double* wout;         // where the split kernel stores the 2N register (w, or a
+                        // second buffer in the deferred-update stage)
   const double* means;  // per-variable offsets of the mixed form (results + R_MEAN0)
   double* partials;     // per-block reduction slots (NRED each) or nullptr
   const double* d_dt;   // device dt (results + R_DT) for Courant-limited runs, else nullptr
@@ -35,6 +37,11 @@ struct KArgs {
   int first;            // itsub == 1: df starts at zero (timestep.f90:110)
   int mode;             // MODE_*
   int write_w;          // store the 2N register (not on the last substep)
+  // kernels with a deferred first-substep update (the RK stage comes in
+  // `stage`): the second substep forms f1 = f0 + dt*beta1*w1 while staging
+  int stage;
+  double beta1;         // beta_ts(1)
+  double dtb1;          // dt*beta_ts(1) when d_dt == nullptr
 };
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -41,17 +41,25 @@
 namespace pcmx {
 
 // RK stages of the 2N scheme (timestep.f90:110-161) as the fused kernel sees
-// them.  S1: first substep with a fixed step (f += dt*beta*w, store w);
-// S2: middle substep (reads and stores w); S3: last substep (reads w, does
-// not store it); W: first substep of a Courant-limited step (stores w only,
-// reduces dt1; the update follows once dt is known); DT: reductions only.
-enum Stage { ST_S1 = 0, ST_S2 = 1, ST_S3 = 2, ST_W = 3, ST_DT = 4 };
+// them.  S1: first substep with a fixed step (store f1 = f0 + dt*beta1*w1 and
+// w1); S2: middle substep (reads own-cell w, stores f and w); S3: last
+// substep (reads w, stores f); DT: reductions only.
+// Courant-limited steps defer the first update: W stores only w1 = RHS(f0)
+// and reduces dt1 (alpha_1 = 0), and S2D stages f1 = f0 + dt*beta1*w1 into
+// the LDS ring as it loads f0 and w1 (the operation of timestep.f90:160, so
+// f1 is bitwise the value S1 would store) and writes w2 to a second buffer
+// (neighbouring blocks still read w1 in their halos).  That replaces the
+// separate update pass an explicit Courant step needs once dt is known:
+// 9 state accesses per variable and step plus the w1 halo, against 12.  With
+// a fixed step the explicit form is cheaper (w1 would be read with halo).
+enum Stage { ST_S1 = 0, ST_S2 = 1, ST_S3 = 2, ST_W = 3, ST_DT = 4, ST_S2D = 5 };
 
 template <int ST>
 struct StageT {
-  static constexpr bool READW = ST == ST_S2 || ST == ST_S3;
-  static constexpr bool WRITEW = ST == ST_S1 || ST == ST_S2 || ST == ST_W;
-  static constexpr bool UPDATE = ST == ST_S1 || ST == ST_S2 || ST == ST_S3;
+  static constexpr bool FORM1 = ST == ST_S2D;
+  static constexpr bool READW = ST == ST_S2 || ST == ST_S2D || ST == ST_S3;
+  static constexpr bool WRITEW = ST == ST_S1 || ST == ST_S2 || ST == ST_S2D || ST == ST_W;
+  static constexpr bool UPDATE = ST == ST_S1 || ST == ST_S2 || ST == ST_S2D || ST == ST_S3;
   static constexpr bool DT = ST == ST_W || ST == ST_DT;
   static constexpr bool DIAG = ST == ST_S1 || ST == ST_W || ST == ST_DT;
 };
@@ -129,7 +137,10 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   const int rl = rv * ZM_PLANE + re;
   const double rmu = MIXED ? a.means[rv] : 0.0;
 
-  double pre[NV * K], prem = 0.0;
+  // dt*beta1 of the deferred first-substep update (stage S2)
+  const double dtb1 = a.d_dt ? a.d_dt[0] * a.beta1 : a.dtb1;
+  constexpr int NW = G::FORM1 ? NV * K : 1;
+  double pre[NV * K], prem = 0.0, prw[NW], prwm = 0.0;
   auto load_plane = [&](int p) {
     const double* src = a.fin + zoff(p);
 #pragma unroll
@@ -139,6 +150,21 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
       for (int k = 0; k < K; ++k) pre[v * K + k] = sv[poff[k]];
     }
     if (has_rem) prem = src[rv * nxyz + roff];
+    if constexpr (G::FORM1) {
+      const double* srw = a.w + zoff(p);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const double* sv = srw + v * nxyz;
+#pragma unroll
+        for (int k = 0; k < K; ++k) prw[v * K + k] = sv[poff[k]];
+      }
+      if (has_rem) prwm = srw[rv * nxyz + roff];
+    }
+  };
+  // the staged value: f (or f1 = f0 + dt*beta1*w1 in stage S2)
+  auto staged = [&](double f, double w) -> double {
+    if constexpr (G::FORM1) return f + dtb1 * w;
+    else return f;
   };
   auto store_plane = [&](int p) {
     T* dst = ring + slot_of(p) * SLOT;
@@ -147,11 +173,14 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
       const double mu = MIXED ? a.means[v] : 0.0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const double val = pre[v * K + k];
+        const double val = staged(pre[v * K + k], prw[G::FORM1 ? v * K + k : 0]);
         dst[v * ZM_PLANE + tid + k * NTOT] = MIXED ? (T)(val - mu) : (T)val;
       }
     }
-    if (has_rem) dst[rl] = MIXED ? (T)(prem - rmu) : (T)prem;
+    if (has_rem) {
+      const double val = staged(prem, prwm);
+      dst[rl] = MIXED ? (T)(val - rmu) : (T)val;
+    }
   };
 
   // the variables of this thread's group: flow vi -> 0,1,2,3,[ISS]; field vi -> IA+vi
@@ -165,7 +194,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   // own-cell prefetch of the next output plane: 2N register (READW) and, in
   // the mixed form, the FP64 state values (the ring holds FP32 offsets)
   constexpr int NOWNF = MIXED ? (MAG ? 4 : 0) : 0;  // field half also needs u, lnrho in FP64
-  double wnx[NGV], fnx[NGV + NOWNF];
+  constexpr int NOWNW = G::FORM1 ? NOWNF : 0;       // ... and, in stage S2, their w1
+  double wnx[NGV + NOWNW], fnx[NGV + NOWNF];
   auto prefetch_own = [&](auto g_c, int p) {
     constexpr int g = decltype(g_c)::value;
     constexpr int ngv = g == 0 ? NF : 3;
@@ -180,6 +210,10 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
       if constexpr (MAG && g == 1) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) fnx[NGV + i] = vplane(a.fin, i, p)[own];
+        if constexpr (G::FORM1) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) wnx[NGV + i] = __builtin_nontemporal_load(&vplane(a.w, i, p)[own]);
+        }
       }
     }
   };
@@ -287,11 +321,21 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     const T* L2 = ring + slot_of(q + 2) * SLOT + c;
     const T* L3 = ring + slot_of(q + 3) * SLOT + c;
     // ---------------- phase 1 ----------------
-    double wcur[NGV], fcur[NGV + NOWNF];
+    double wcur[NGV + NOWNW], fcur[NGV + NOWNF];
 #pragma unroll
-    for (int i = 0; i < NGV; ++i) wcur[i] = wnx[i];
+    for (int i = 0; i < NGV + NOWNW; ++i) wcur[i] = wnx[i];
 #pragma unroll
     for (int i = 0; i < NGV + NOWNF; ++i) fcur[i] = fnx[i];
+    if constexpr (MIXED && G::FORM1) {
+      // own-cell FP64 state of the mixed form: f1 = f0 + dt*beta1*w1
+      constexpr int ngv = GRP == 0 ? NF : 3;
+#pragma unroll
+      for (int i = 0; i < ngv; ++i) fcur[i] = fcur[i] + dtb1 * wcur[i];
+      if constexpr (GRP == 1) {
+#pragma unroll
+        for (int i = 0; i < NOWNW; ++i) fcur[NGV + i] = fcur[NGV + i] + dtb1 * wcur[NGV + i];
+      }
+    }
     if constexpr (GRP == 0) {
       Derivs<T> d;
 #pragma unroll
@@ -352,7 +396,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
-          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(wa[i], &vplane(a.w, IA + i, q)[own]);
+          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(wa[i], &vplane(a.wout, IA + i, q)[own]);
         }
         if constexpr (G::DIAG) {
           red[2] += b2;
@@ -381,7 +425,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
-          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(wf[i], &vplane(a.w, var_of(0, i), q)[own]);
+          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(wf[i], &vplane(a.wout, var_of(0, i), q)[own]);
         }
         if constexpr (G::DIAG) {
           red[0] += u2c;
