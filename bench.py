@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Throughput of the fused RHS + 2N-RK3 step on one MI355X.
 
-Workload (default): isothermal MHD, 512^3 periodic box, 7 FP64 variables
-(uu, lnrho, aa), nu = eta = 1e-3, cs = 1, fixed dt = 1e-3, synthetic seeded
-Gaussian-noise state (seed 7).  A step is one full RK3 step (3 fused kernel
-launches).  The timed region runs --steps steps with the state resident in
-HBM, bracketed by device synchronisation.  Reported:
-  value       Mcell-updates/s of the FP64 form (interior cells x steps / s)
-  roofline    the fused kernel's achieved algorithmic HBM bandwidth
-              (80 B per variable per cell-update, averaged over the 3
-              launches of a step) against 8 TB/s, timed with HIP events
-  mixed       the same for the mixed FP32-on-offset form
+Default workload: isothermal MHD, 512^3 periodic box of side 2pi, 7 FP64
+variables (uu, lnrho, aa), nu = eta = 1e-3, cs = 1, mu0 = 1, fixed
+dt = 1e-3, synthetic state: uu and aa band-limited random fields on the
+shells 1 <= |k| <= 4 (rms 0.05 and 0.02), lnrho = 0, seed 7.  A step is one
+full RK3 step (3 fused kernel launches).  The timed region runs --steps steps
+with the state resident in HBM, bracketed by device synchronisation.
+Reported on one JSON line:
+  value        Mcell-updates/s of the FP64 form (interior cells x steps / s)
+  roofline     the fused kernel's achieved algorithmic HBM bandwidth (80 B
+               per variable per cell-update, averaged over the 3 launches of
+               a step) against 8 TB/s, timed with HIP events on the library
+               stream; `traffic` is the measured L2<->memory bytes per launch
+               from the committed rocprofv3 counter profile of this workload
+  mixed        the same for the mixed FP32-on-offset form
   cpu_baseline the CPU oracle (C port of the reference pencil loop, OpenMP)
-              on a bounded sample of the same physics on this host
-Prints exactly one JSON line on stdout.
+               on a bounded sample of the same physics on this host
+
+--config K instead runs one of the reference configurations and prints its
+throughput and the per-variable errors (FP64 against the CPU oracle where the
+oracle finishes in under a minute, the mixed form against the FP64 form).
 """
 import argparse
 import json
@@ -32,34 +39,81 @@ import pcmx_ics  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VAR = 80  # 10 FP64 state accesses per variable per RK3 step
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rhs_zsplit_traffic_mhd512.json")
+METRIC = "Mcell-updates/s per RK3 step, 512^3 MHD, FP64 vs mixed; % of HBM peak"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(n, nvar_mag, precision, steps, warmup, dt, kernel, f):
-    par = pcmx.make_params(n, lmagnetic=nvar_mag, nu=1e-3, eta=1e-3, precision=precision, kernel=kernel)
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def rel_linf(a, b):
+    out = []
+    for v in range(a.shape[0]):
+        den = np.abs(b[v]).max()
+        out.append(float(np.abs(a[v] - b[v]).max() / den) if den > 0 else float(np.abs(a[v]).max()))
+    return out
+
+
+def run_form(f, par, steps, warmup, dt, keep_state=False):
+    """warmup + timed steps of one engine; returns timings and, optionally,
+    the final interior state."""
     pcmx.initialize_gpu(f, par, nt=1)
     try:
         t = 0.0
         if warmup:
-            t, _ = pcmx.time_step(warmup, t, dt, ldiag=True)
+            t, dt = pcmx.time_step(warmup, t, dt, ldiag=True)
         pcmx.set_timing(True)
         t0 = time.perf_counter()
-        t, _ = pcmx.time_step(steps, t, dt, ldiag=True)   # synchronises at the end
+        t, dt = pcmx.time_step(steps, t, dt, ldiag=True)   # synchronises at the end
         sec = time.perf_counter() - t0
         kms, nk = pcmx.kernel_timing()
         red = pcmx.get_gpu_reduced_vars()
+        state = None
+        if keep_state:
+            out = np.zeros_like(f)
+            pcmx.copy_farray_from_gpu(out)
+            state = pcmx.interior_of(out, par.mvar).copy()
     finally:
         pcmx.finalize_gpu()
-    ncell = float(n) ** 3
-    value = ncell * steps / sec / 1e6
-    nvar = par.mvar
-    bytes_per_launch = BYTES_PER_VAR * nvar * ncell / 3.0
+    ncell = float(par.nx) * par.ny * par.nz
+    bytes_per_launch = BYTES_PER_VAR * par.mvar * ncell / 3.0
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    return dict(value=value, sec=sec, ms_per_step=sec * 1e3 / steps, kernel_ms=kms, nkernels=nk,
-                achieved=achieved, urms=float(red[0]), brms=float(red[2]))
+    return dict(value=ncell * steps / sec / 1e6, sec=sec, ms_per_step=sec * 1e3 / steps, kernel_ms=kms,
+                nkernels=nk, achieved=achieved, urms=float(red[0]), brms=float(red[2]), t=t, dt=dt,
+                state=state)
+
+
+def measured_traffic(n):
+    """L2<->memory bytes per launch of the default workload, from the
+    committed rocprofv3 counter profile (None if absent or another grid)."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return d.get("traffic_bytes_per_launch") if d.get("grid") == n else None
+
+
+def cpu_baseline(threads):
+    """Oracle (C port of the reference pencil loop) on a bounded sample."""
+    O = oracle()
+    n, steps = 256, 30
+    p = O.make_params(n, lmagnetic=True, nu=1e-3, eta=1e-3, dt=1e-3)
+    f0 = pcmx_ics.config_state("shells", n, 7, lmagnetic=True)
+    O.advance(f0, p, 1, nthreads=threads)          # warm caches / threads
+    t0 = time.perf_counter()
+    O.advance(f0, p, steps, nthreads=threads)
+    sec = time.perf_counter() - t0
+    return dict(value=n ** 3 * steps / sec / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
+                sample=f"isothermal MHD {n}^3, {steps} RK3 steps, FP64, oracle pencil loop on {threads} "
+                       f"OpenMP threads ({sec:.1f} s)")
 
 
 def copy_calibration(mib, reps):
@@ -86,21 +140,63 @@ def copy_calibration(mib, reps):
     log(f"bench: copy calibration {reps} x {mib} MiB device-to-device")
 
 
-def cpu_baseline(threads):
-    """Oracle (C port of the reference pencil loop) on a bounded sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+# ---------------------------------------------------------------- configs
+# The reference configurations (grid, variables, transport coefficients,
+# initial state, step control, number of steps).
+CONFIGS = {
+    1: dict(desc="isothermal hydro 32^3, 4 vars, nu=5e-3, fixed dt=1e-3, velocity noise 1e-3, 100 steps",
+            n=32, mag=False, ent=False, nu=5e-3, eta=0.0, chi=0.0, gamma=1.0, dt=1e-3, ldt=False,
+            ic="noise", seed=1, steps=100, oracle_steps=100),
+    2: dict(desc="isothermal MHD 128^3, 7 vars, nu=eta=5e-3, fixed dt=2e-3, band-limited uu/aa, 100 steps",
+            n=128, mag=True, ent=False, nu=5e-3, eta=5e-3, chi=0.0, gamma=1.0, dt=2e-3, ldt=False,
+            ic="shells", seed=2, steps=100, oracle_steps=100),
+    3: dict(desc="MHD + entropy 256^3, 8 vars, gamma=5/3, nu=eta=chi=2e-3, Courant dt (cdt=0.4), "
+                 "band-limited uu/aa + ss noise 1e-3, 100 steps",
+            n=256, mag=True, ent=True, nu=2e-3, eta=2e-3, chi=2e-3, gamma=5 / 3, dt=0.0, ldt=True, cdt=0.4,
+            ic="shells", seed=3, steps=100, oracle_steps=10),
+    4: dict(desc="isothermal MHD 512^3, 7 vars, nu=eta=1e-3, fixed dt=1e-3, band-limited uu/aa (seed 7), "
+                 "50 steps", n=512, mag=True, ent=False, nu=1e-3, eta=1e-3, chi=0.0, gamma=1.0, dt=1e-3,
+            ldt=False, ic="shells", seed=7, steps=50, oracle_steps=0),
+}
 
-    n, steps = 256, 30
-    p = O.make_params(n, lmagnetic=True, nu=1e-3, eta=1e-3, dt=1e-3)
-    f0 = pcmx_ics.config_state("noise", n, 7, lmagnetic=True)
-    O.advance(f0, p, 1, nthreads=threads)          # warm caches / threads
-    t0 = time.perf_counter()
-    O.advance(f0, p, steps, nthreads=threads)
-    sec = time.perf_counter() - t0
-    return dict(value=n ** 3 * steps / sec / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
-                sample=f"isothermal MHD {n}^3, {steps} RK3 steps, FP64, oracle pencil loop on {threads} "
-                       f"OpenMP threads ({sec:.1f} s)")
+
+def run_config(k, threads):
+    c = CONFIGS[k]
+    n = c["n"]
+    log(f"bench: config {k}: {c['desc']}")
+    f0 = pcmx_ics.config_state(c["ic"], n, c["seed"], lmagnetic=c["mag"], lentropy=c["ent"])
+    f = pcmx.farray_from_interior(f0)
+    kw = dict(lmagnetic=c["mag"], lentropy=c["ent"], nu=c["nu"], eta=c["eta"], chi=c["chi"], gamma=c["gamma"],
+              ldt=c["ldt"], cdt=c.get("cdt", 0.9))
+    out = {"config": k, "workload": c["desc"], "unit": "Mcell-updates/s"}
+    res = {}
+    for name, prec in (("fp64", pcmx.PREC_FP64), ("mixed", pcmx.PREC_MIXED)):
+        par = pcmx.make_params(n, precision=prec, **kw)
+        r = run_form(f, par, c["steps"], 0, c["dt"], keep_state=True)
+        res[name] = r
+        out[name] = {"value": round(r["value"], 2), "kernel_ms": round(r["kernel_ms"], 4),
+                     "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "t": r["t"]}
+        log(f"bench: config {k} {name}: {out[name]}")
+    out["mixed_vs_fp64_rel_linf"] = rel_linf(res["mixed"]["state"], res["fp64"]["state"])
+    if c["oracle_steps"]:
+        O = oracle()
+        m = c["oracle_steps"]
+        op = O.make_params(n, lmagnetic=c["mag"], lentropy=c["ent"], nu=c["nu"], eta=c["eta"], chi=c["chi"],
+                           gamma=c["gamma"], cdt=c.get("cdt", 0.9), dt=0.0 if c["ldt"] else c["dt"])
+        t0 = time.perf_counter()
+        o, to, _, _ = O.advance(f0, op, m, nthreads=threads)
+        osec = time.perf_counter() - t0
+        g = {}
+        for name, prec in (("fp64", pcmx.PREC_FP64), ("mixed", pcmx.PREC_MIXED)):
+            par = pcmx.make_params(n, precision=prec, **kw)
+            g[name] = run_form(f, par, m, 0, c["dt"], keep_state=True)
+        out["oracle_steps"] = m
+        out["fp64_vs_oracle_rel_linf"] = rel_linf(g["fp64"]["state"], o)
+        out["mixed_vs_oracle_rel_linf"] = rel_linf(g["mixed"]["state"], o)
+        out["t_oracle"] = to
+        out["oracle_sec"] = round(osec, 2)
+        out["oracle_value"] = round(float(n) ** 3 * m / osec / 1e6, 2)
+    return out
 
 
 def main():
@@ -113,6 +209,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=None)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="run reference configuration K instead of the default bench line")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run 4 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -120,33 +218,38 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
+    threads = min(16, os.cpu_count() or 1)
+    if args.config:
+        print(json.dumps(run_config(args.config, threads)), flush=True)
+        return
 
     n = args.n
-    log(f"bench: building {n}^3 isothermal MHD state (seed 7)")
-    interior = pcmx_ics.config_state("noise", n, 7, lmagnetic=True)
+    log(f"bench: building {n}^3 isothermal MHD state (band-limited uu/aa, seed 7)")
+    interior = pcmx_ics.config_state("shells", n, 7, lmagnetic=True)
     f = pcmx.farray_from_interior(interior)
     del interior
 
+    kw = dict(lmagnetic=True, nu=1e-3, eta=1e-3, kernel=args.kernel)
     log("bench: FP64 form")
-    r64 = run_form(n, True, pcmx.PREC_FP64, args.steps, args.warmup, args.dt, args.kernel, f)
-    log(f"bench: FP64 {r64}")
+    r64 = run_form(f, pcmx.make_params(n, precision=pcmx.PREC_FP64, **kw), args.steps, args.warmup, args.dt)
+    log(f"bench: FP64 { {k: v for k, v in r64.items() if k != 'state'} }")
     rmx = None
     if not args.no_mixed:
         log("bench: mixed form")
-        rmx = run_form(n, True, pcmx.PREC_MIXED, args.steps, args.warmup, args.dt, args.kernel, f)
-        log(f"bench: mixed {rmx}")
+        rmx = run_form(f, pcmx.make_params(n, precision=pcmx.PREC_MIXED, **kw), args.steps, args.warmup,
+                       args.dt)
+        log(f"bench: mixed { {k: v for k, v in rmx.items() if k != 'state'} }")
     del f
     if args.copy_calib:
         copy_calibration(args.copy_calib, 4)
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {threads} threads")
         cpu = cpu_baseline(threads)
         log(f"bench: cpu {cpu}")
 
     out = {
-        "metric": "Mcell-updates/s per RK3 step, 512^3 MHD, FP64 vs mixed; % of HBM peak",
+        "metric": METRIC,
         "value": round(r64["value"], 3),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
@@ -159,10 +262,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"isothermal MHD {n}^3 periodic, 7 FP64 vars, nu=eta=1e-3, cs=1, "
-                               f"fixed dt={args.dt}, seeded Gaussian noise IC (seed 7)",
+                               f"fixed dt={args.dt}, band-limited uu/aa (1<=|k|<=4, rms 0.05/0.02), "
+                               "lnrho=0, seed 7",
                    "grid": n, "nvar": 7, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r64["achieved"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r64["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(r64["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(n),
                      "kernel_ms": round(r64["kernel_ms"], 4),
                      "bytes_per_launch": BYTES_PER_VAR * 7 * n ** 3 / 3.0},
         "cpu_baseline": cpu,

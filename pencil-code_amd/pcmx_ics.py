@@ -6,7 +6,8 @@
   seed, not from the reference's mars_ran generator.
 * ``bandlimited``: a random vector field whose Fourier modes are confined to
   the shell kmin <= |k| <= kmax (integer wavenumbers of the 2pi box), scaled
-  to a given rms, as the isothermal-MHD cases ask for.
+  to a given rms, as the isothermal-MHD cases ask for (the box-mean mode is
+  outside every shell, so the field has zero mean).
 """
 from __future__ import annotations
 
@@ -20,18 +21,25 @@ def gaussian_noise(ncomp, n, ampl, seed):
 
 
 def bandlimited(ncomp, n, kmin, kmax, rms, seed):
+    """Random field with Fourier modes only on the shell kmin <= |k| <= kmax:
+    i.i.d. complex normal coefficients on the shell modes of the half
+    spectrum, one inverse real FFT per component (the same statistics as
+    shell-filtered white noise, without the forward transform), scaled so
+    that sqrt(<|f|^2>) = rms."""
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     rng = np.random.default_rng(seed)
-    kx = np.fft.fftfreq(nx, 1.0 / nx)[None, None, :]
+    kx = np.arange(nx // 2 + 1)[None, None, :]
     ky = np.fft.fftfreq(ny, 1.0 / ny)[None, :, None]
     kz = np.fft.fftfreq(nz, 1.0 / nz)[:, None, None]
     kk = np.sqrt(kx * kx + ky * ky + kz * kz)
-    mask = (kk >= kmin) & (kk <= kmax)
+    idx = np.nonzero((kk >= kmin) & (kk <= kmax))
     out = np.empty((ncomp, nz, ny, nx))
+    spec = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128)
     for c in range(ncomp):
-        noise = rng.standard_normal((nz, ny, nx))
-        spec = np.fft.fftn(noise) * mask
-        out[c] = np.real(np.fft.ifftn(spec))
+        m = idx[0].size
+        spec[idx] = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+        out[c] = np.fft.irfftn(spec, s=(nz, ny, nx))
+        spec[idx] = 0
     cur = np.sqrt(np.mean(np.sum(out * out, axis=0)))
     return out * (rms / cur) if cur > 0 else out
 
