@@ -199,6 +199,83 @@ def run_config(k, threads):
     return out
 
 
+ENSEMBLE_DESC = ("ensemble of 64 isothermal MHD boxes, sizes uniform in {32,48,64,96}^3 (side 2pi), "
+                 "eta log-uniform in [1e-3,1e-2], Pm=nu/eta log-uniform in [0.5,10], band-limited uu/aa with "
+                 "distinct seeds, fixed dt per box, 200 steps; mixed form = FP64 state with FP32 RHS")
+
+
+def ensemble_setup(nbox=64, seed=5):
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([32, 48, 64, 96], size=nbox)
+    boxes, states = [], []
+    for i, n in enumerate(sizes):
+        eta = float(10 ** rng.uniform(-3, -2))
+        nu = float(eta * 10 ** rng.uniform(math.log10(0.5), 1.0))
+        dx = 2 * math.pi / n
+        dt = min(2e-3, 0.1 * dx * dx / max(nu, eta))   # well inside the diffusive limit
+        boxes.append(pcmx.make_box(int(n), nu, eta, dt))
+        states.append(pcmx_ics.config_state("shells", int(n), 1000 + i, lmagnetic=True))
+    return boxes, states
+
+
+def run_ensemble(boxes, states, precision, steps, tune=True):
+    par = pcmx.make_params(32, lmagnetic=True, precision=precision)
+    pcmx.ens_create(par, boxes)
+    try:
+        for i, f in enumerate(states):
+            pcmx.ens_load(i, f)
+        if tune:
+            pcmx.ens_tune(2)
+        pcmx.ens_step(2, ldiag=True)                       # warm-up (graphs built)
+        for i, f in enumerate(states):
+            pcmx.ens_load(i, f)
+        pcmx.ens_set_timing(True)
+        t0 = time.perf_counter()
+        pcmx.ens_step(steps, ldiag=True)
+        sec = time.perf_counter() - t0
+        info = pcmx.ens_timing()
+        out = [pcmx.ens_copy(i, f.shape) for i, f in enumerate(states)]
+    finally:
+        pcmx.ens_destroy()
+    return sec, info, out
+
+
+def run_config5(threads):
+    steps = 200
+    log(f"bench: config 5: {ENSEMBLE_DESC}")
+    boxes, states = ensemble_setup()
+    ncell = sum(b.nx * b.ny * b.nz for b in boxes)
+    out = {"config": 5, "workload": ENSEMBLE_DESC, "unit": "Mcell-updates/s", "cells": ncell}
+    res = {}
+    for name, prec in (("mixed", pcmx.PREC_MIXED), ("fp64", pcmx.PREC_FP64)):
+        sec, info, st = run_ensemble(boxes, states, prec, steps)
+        res[name] = st
+        bytes_per_launch = BYTES_PER_VAR * 7 * ncell / 3.0
+        out[name] = {"value": round(ncell * steps / sec / 1e6, 2), "ms_per_step": round(sec * 1e3 / steps, 4),
+                     "substep_ms": round(info[0], 4),
+                     "roofline_frac": round(bytes_per_launch / (info[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "plan": {"tile": f"{info[2]}x{256 // info[2]}", "lz_max": info[3], "blocks_per_cu": info[4],
+                              "work_items": info[5]}}
+        log(f"bench: config 5 {name}: {out[name]}")
+    err = np.zeros(7)
+    for a, b in zip(res["mixed"], res["fp64"]):
+        for v in range(7):
+            den = np.abs(b[v]).max()
+            err[v] = max(err[v], np.abs(a[v] - b[v]).max() / den if den > 0 else 0.0)
+    out["mixed_vs_fp64_rel_linf"] = err.tolist()
+    # the same boxes one after another through the single-box engine
+    m = 16
+    t = 0.0
+    for b, f in zip(boxes[:m], states[:m]):
+        par = pcmx.make_params((b.nx, b.ny, b.nz), lmagnetic=True, nu=b.nu, eta=b.eta)
+        r = run_form(pcmx.farray_from_interior(f), par, 20, 2, b.dt)
+        t += r["sec"]
+    cells_m = sum(b.nx * b.ny * b.nz for b in boxes[:m])
+    out["one_box_at_a_time"] = {"value": round(cells_m * 20 / t / 1e6, 2),
+                                "sample": f"first {m} boxes, 20 steps each, FP64 single-box engine"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,7 +286,7 @@ def main():
     ap.add_argument("--kernel", type=int, default=None)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", type=int, default=0, choices=[0, 1, 2, 3, 4],
+    ap.add_argument("--config", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
                     help="run reference configuration K instead of the default bench line")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run 4 device-to-device copies of MIB MiB (counter calibration)")
@@ -220,7 +297,8 @@ def main():
         sys.exit(2)
     threads = min(16, os.cpu_count() or 1)
     if args.config:
-        print(json.dumps(run_config(args.config, threads)), flush=True)
+        res = run_config5(threads) if args.config == 5 else run_config(args.config, threads)
+        print(json.dumps(res), flush=True)
         return
 
     n = args.n

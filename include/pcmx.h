@@ -143,6 +143,44 @@ int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag);
 int pcmx_kernel_timing(double* out);
 int pcmx_set_timing(int enable);
 
+/* ------------------------------------------------------------------------
+ * Ensembles of independent boxes.  No counterpart in the reference's module
+ * GPU (one box per run there); for callers that advance many small periodic
+ * boxes, e.g. parameter scans.  All boxes share the physics of the params
+ * passed to pcmx_ens_create (variables, EOS, precision: isothermal hydro or
+ * MHD, fixed steps); each box has its own grid, spacing, viscosity,
+ * resistivity and time step.  States are exchanged interior-only, as
+ * f[var][z][y][x] doubles (C order), var in registration order.  One
+ * ensemble per process, independent of the single-box engine above. */
+typedef struct pcmx_box {
+  int nx, ny, nz;      /* interior grid; nx, ny multiples of 16 (or of 32 and 8) */
+  double dx, dy, dz;   /* grid spacing */
+  double nu, eta;      /* viscosity, resistivity */
+  double dt;           /* fixed time step */
+} pcmx_box;
+
+#define PCMX_ENS_NO_GRAPH 1 /* flags: advance with plain launches instead of a HIP graph */
+
+int pcmx_ens_create(const pcmx_params* par, const pcmx_box* boxes, int nbox, int flags);
+int pcmx_ens_destroy(void);
+/* Load / copy back the interior state of box ibox (mvar*nz*ny*nx doubles). */
+int pcmx_ens_load(int ibox, const double* f);
+int pcmx_ens_copy(int ibox, double* f);
+/* Advance every box nsteps RK3 steps (one persistent launch per substep;
+ * pairs of steps replayed from a captured HIP graph).  With ldiag, urms,
+ * umax, brms of each box are reduced on every step's first substep. */
+int pcmx_ens_step(int nsteps, int ldiag);
+/* dst[3*ibox + 0..2] = urms, umax, brms of the last diagnostic substep. */
+int pcmx_ens_reduced(double* dst);
+/* Times candidate tile shapes, z-chunk lengths and blocks per CU on the
+ * loaded state (nsteps_probe steps each, the state restored afterwards) and
+ * keeps the fastest. */
+int pcmx_ens_tune(int nsteps_probe);
+int pcmx_ens_set_timing(int enable);
+/* out[0] mean ms per substep of the last timed pcmx_ens_step, out[1] substeps,
+ * out[2..5] tile width, z-chunk limit, blocks per CU, work items. */
+int pcmx_ens_timing(double* out);
+
 #ifdef __cplusplus
 }
 #endif

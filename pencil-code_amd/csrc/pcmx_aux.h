@@ -1,0 +1,91 @@
+// pcmx_aux.h -- the small kernels around the RHS kernels: the reduction
+// finaliser, the explicit first-substep update for Courant-limited steps, and
+// the layout conversion between the reference f-array (with ghost zones) and
+// device state.  Included by the host translation unit only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pcmx_kernels.h"
+
+namespace pcmx {
+
+// Finalises the block partials in a fixed order (deterministic): urms, umax,
+// brms (diagnostics.f90 sum_mn_name / max_mn_name with lsqrt, dVol_rel1 =
+// 1/nwgrid at :274), dt1_max, and when set_dt the step dt = 1/dt1_max
+// (sub.f90:4223-4254), and the box means.  One block of 256 threads.
+__global__ void __launch_bounds__(256) k_finalize(const double* partials, long nblk, double* results,
+                                                  double ncells, int mvar, int set_dt, int set_means) {
+  __shared__ double lds[4][NRED];
+  double r[NRED];
+#pragma unroll
+  for (int q = 0; q < NRED; ++q) r[q] = 0.0;
+  for (long b = threadIdx.x; b < nblk; b += 256) {
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) {
+      const double v = partials[b * NRED + q];
+      r[q] = (q == 1 || q == 3) ? fmax(r[q], v) : r[q] + v;
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NRED; ++q) {
+    const double v = (q == 1 || q == 3) ? wave_max(r[q]) : wave_sum(r[q]);
+    if (lane == 0) lds[wave][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[NRED];
+    for (int q = 0; q < NRED; ++q) {
+      t[q] = lds[0][q];
+      for (int wv = 1; wv < 4; ++wv) t[q] = (q == 1 || q == 3) ? fmax(t[q], lds[wv][q]) : t[q] + lds[wv][q];
+    }
+    results[R_URMS] = sqrt(t[0] / ncells);
+    results[R_UMAX] = sqrt(t[1]);
+    results[R_BRMS] = sqrt(t[2] / ncells);
+    results[R_DT1] = t[3];
+    if (set_dt) results[R_DT] = 1.0 / t[3];
+    if (set_means)
+      for (int v = 0; v < mvar; ++v) results[R_MEAN0 + v] = t[4 + v] / ncells;
+  }
+}
+
+// Deferred update of the first substep when dt comes from the Courant
+// condition of that substep's RHS: f_out = f_in + (dt*beta)*w
+// (timestep.f90:150,160-161).  Grid-stride over mvar*nxyz elements.
+__global__ void __launch_bounds__(256) k_axpy_dt(const double* __restrict__ fin, double* __restrict__ fout,
+                                                 const double* __restrict__ w, const double* d_dt, double beta,
+                                                 long n) {
+  const double dtb = d_dt[0] * beta;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) fout[i] = fin[i] + dtb * w[i];
+}
+
+// Device interior -> one variable of the reference f-array with periodic
+// ghost zones (bc_per_x/y/z, boundcond.f90:2387-2433,2509-).
+__global__ void __launch_bounds__(256) k_pack_ghosts(const double* __restrict__ src, double* __restrict__ dst,
+                                                     int nx, int ny, int nz) {
+  const int mx = nx + 6, my = ny + 6, mz = nz + 6;
+  const long mw = (long)mx * my * mz;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < mw; i += (long)gridDim.x * 256L) {
+    const int ix = (int)(i % mx);
+    const long r = i / mx;
+    const int iy = (int)(r % my), iz = (int)(r / my);
+    const int x = (ix - 3 + nx) % nx, y = (iy - 3 + ny) % ny, z = (iz - 3 + nz) % nz;
+    dst[i] = src[((long)z * ny + y) * nx + x];
+  }
+}
+
+// One variable of the reference f-array -> device interior.
+__global__ void __launch_bounds__(256) k_unpack_interior(const double* __restrict__ src, double* __restrict__ dst,
+                                                         int nx, int ny, int nz) {
+  const int mx = nx + 6, my = ny + 6;
+  const long n = (long)nx * ny * nz;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+    const int x = (int)(i % nx);
+    const long r = i / nx;
+    const int y = (int)(r % ny), z = (int)(r / ny);
+    dst[i] = src[((long)(z + 3) * my + (y + 3)) * mx + (x + 3)];
+  }
+}
+
+}  // namespace pcmx

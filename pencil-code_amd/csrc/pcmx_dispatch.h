@@ -1,0 +1,85 @@
+// pcmx_dispatch.h -- template dispatch from a LaunchCfg to the kernel
+// variants; included by the kernel translation units, each of which
+// instantiates it for one precision.
+#pragma once
+
+#include "pcmx_direct.h"
+#include "pcmx_launch.h"
+#include "pcmx_zmarch.h"
+#include "pcmx_zsplit.h"
+
+namespace pcmx {
+
+template <typename T, bool MAG, bool ENT, bool DT>
+void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
+  dim3 blk(64, 4, 1), grd((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
+  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT>), grd, blk, 0, s, a, P);
+}
+
+template <typename T, bool MAG, bool ENT, class TL>
+bool launch_split(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
+  switch (a.stage) {
+    case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1, TL>(a, P, zp, s); return true;
+    case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2, TL>(a, P, zp, s); return true;
+    case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3, TL>(a, P, zp, s); return true;
+    case ST_W: launch_zsplit<T, MAG, ENT, ST_W, TL>(a, P, zp, s); return true;
+    case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D, TL>(a, P, zp, s); return true;
+    case ST_DT: launch_zsplit<T, MAG, ENT, ST_DT, TL>(a, P, zp, s); return true;
+    default: return false;
+  }
+}
+
+template <typename T, bool MAG, bool ENT>
+bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  if (c.kernel == 3) {
+    if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16>(a, P, c.zp, s);
+    return launch_split<T, MAG, ENT, ZT32x8>(a, P, c.zp, s);
+  }
+  if (c.kernel == 2) {
+    if (c.dt) launch_zmarch<T, MAG, ENT, true>(a, P, c.zp, s);
+    else launch_zmarch<T, MAG, ENT, false>(a, P, c.zp, s);
+    return true;
+  }
+  if (c.dt) launch_direct<T, MAG, ENT, true>(a, P, s);
+  else launch_direct<T, MAG, ENT, false>(a, P, s);
+  return true;
+}
+
+template <typename T>
+bool launch_rhs_t(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  if (c.mag) return c.ent ? launch_phys<T, true, true>(c, a, P, s) : launch_phys<T, true, false>(c, a, P, s);
+  return c.ent ? launch_phys<T, false, true>(c, a, P, s) : launch_phys<T, false, false>(c, a, P, s);
+}
+
+// ---------------------------------------------------------------- ensemble
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+void launch_ens_k(const EnsBox* boxes, const EnsWork* work, int nwork, double* partials, int nblocks,
+                  hipStream_t s) {
+  hipLaunchKernelGGL((k_rhs_zsplit_ens<T, MAG, ENT, POLY, ST, TL>), dim3((unsigned)nblocks),
+                     dim3(zs_threads<MAG>()), 0, s, boxes, work, nwork, partials);
+}
+
+template <typename T, bool MAG, bool ENT, bool POLY, class TL>
+bool launch_ens_st(int stage, const EnsBox* b, const EnsWork* w, int nw, double* pr, int nb, hipStream_t s) {
+  switch (stage) {
+    case ST_S1: launch_ens_k<T, MAG, ENT, POLY, ST_S1, TL>(b, w, nw, pr, nb, s); return true;
+    case ST_S2: launch_ens_k<T, MAG, ENT, POLY, ST_S2, TL>(b, w, nw, pr, nb, s); return true;
+    case ST_S3: launch_ens_k<T, MAG, ENT, POLY, ST_S3, TL>(b, w, nw, pr, nb, s); return true;
+    case ST_DT: launch_ens_k<T, MAG, ENT, POLY, ST_DT, TL>(b, w, nw, pr, nb, s); return true;
+    default: return false;  // Courant-limited ensembles are not provided
+  }
+}
+
+// The ensemble provides isothermal hydro and MHD (fixed steps).
+template <typename T>
+bool launch_ens_t(const LaunchCfg& c, int stage, const EnsBox* b, const EnsWork* w, int nw, double* pr, int nb,
+                  hipStream_t s) {
+  if (c.ent) return false;
+  if (c.tile == TILE_16x16)
+    return c.mag ? launch_ens_st<T, true, false, false, ZT16x16>(stage, b, w, nw, pr, nb, s)
+                 : launch_ens_st<T, false, false, false, ZT16x16>(stage, b, w, nw, pr, nb, s);
+  return c.mag ? launch_ens_st<T, true, false, false, ZT32x8>(stage, b, w, nw, pr, nb, s)
+               : launch_ens_st<T, false, false, false, ZT32x8>(stage, b, w, nw, pr, nb, s);
+}
+
+}  // namespace pcmx

@@ -20,18 +20,16 @@
 #include <vector>
 
 #include "../../include/pcmx.h"
-#include "pcmx_direct.h"
-#include "pcmx_kernels.h"
-#include "pcmx_zmarch.h"
-#include "pcmx_zsplit.h"
+#include "pcmx_aux.h"
+#include "pcmx_hostutil.h"
+#include "pcmx_launch.h"
 
 using namespace pcmx;
 
-namespace {
+namespace pcmx {
 
 thread_local char g_err[1024] = "";
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -40,44 +38,12 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHECK(call)                                                                        \
-  do {                                                                                        \
-    hipError_t e_ = (call);                                                                   \
-    if (e_ != hipSuccess) return fail(PCMX_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
+const char* last_error() { return g_err; }
 
 // Williamson (1980) 2N-RK3, timestep.f90:43-44.
 const double ALPHA_TS[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
 const double BETA_TS[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
 
-struct Engine {
-  pcmx_params par{};
-  Phys phys{};
-  long nxyz = 0;
-  int nvar = 0;
-  bool mag = false, ent = false;
-  double* f[2] = {nullptr, nullptr};
-  double* w = nullptr;
-  double* w2 = nullptr;       // second 2N register of the deferred Courant step (kernel 3)
-  int cur = 0;
-  double* partials = nullptr;
-  long partials_cap = 0;
-  double* results = nullptr;  // R_COUNT doubles
-  double* staging = nullptr;  // one variable with ghosts (mw doubles)
-  hipStream_t stream = nullptr;
-  int kernel = 1;             // resolved kernel variant
-  ZPlan zplan{};
-  // timing of the fused kernel (pcmx_time_step)
-  bool timing = false;
-  double timing_ms = 0.0;
-  long timing_n = 0;
-  double last_dt = 0.0;
-  // kernel 3 defers the first substep's update to the second substep: after
-  // substep 1, f[cur] still holds f0 and f1 = f0 + dt*beta1*w1 is implicit
-  bool pending_f1 = false;
-};
-
-Engine* g = nullptr;
 
 int check_params(const pcmx_params* p) {
   if (!p) return fail(PCMX_ERR_ARG, "params is NULL");
@@ -137,50 +103,61 @@ Phys make_phys(const pcmx_params& p) {
   return P;
 }
 
+}  // namespace pcmx
+
+namespace {
+
+struct Engine {
+  pcmx_params par{};
+  Phys phys{};
+  long nxyz = 0;
+  int nvar = 0;
+  bool mag = false, ent = false;
+  double* f[2] = {nullptr, nullptr};
+  double* w = nullptr;
+  double* w2 = nullptr;       // second 2N register of the deferred Courant step (kernel 3)
+  int cur = 0;
+  double* partials = nullptr;
+  long partials_cap = 0;
+  double* results = nullptr;  // R_COUNT doubles
+  double* staging = nullptr;  // one variable with ghosts (mw doubles)
+  hipStream_t stream = nullptr;
+  int kernel = 1;             // resolved kernel variant
+  int tile = TILE_32x8;       // tile of the split kernel
+  ZPlan zplan{};
+  // timing of the fused kernel (pcmx_time_step)
+  bool timing = false;
+  double timing_ms = 0.0;
+  long timing_n = 0;
+  double last_dt = 0.0;
+  // kernel 3 defers the first substep's update to the second substep: after
+  // substep 1, f[cur] still holds f0 and f1 = f0 + dt*beta1*w1 is implicit
+  bool pending_f1 = false;
+};
+
+Engine* g = nullptr;
+
 // ---------------------------------------------------------------- launches
-template <typename T, bool MAG, bool ENT, bool DT>
-void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
-  dim3 blk(64, 4, 1), grd((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
-  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT>), grd, blk, 0, s, a, P);
-}
-long direct_blocks(const pcmx_params& p) { return (long)((p.nx + 63) / 64) * ((p.ny + 3) / 4) * p.nz; }
-
-template <typename T, bool MAG, bool ENT>
-void launch_t(const Engine& e, const KArgs& a, bool dt) {
-  if (e.kernel == 3) {
-    switch (a.stage) {
-      case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1>(a, e.phys, e.zplan, e.stream); break;
-      case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2>(a, e.phys, e.zplan, e.stream); break;
-      case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3>(a, e.phys, e.zplan, e.stream); break;
-      case ST_W: launch_zsplit<T, MAG, ENT, ST_W>(a, e.phys, e.zplan, e.stream); break;
-      case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D>(a, e.phys, e.zplan, e.stream); break;
-      default: launch_zsplit<T, MAG, ENT, ST_DT>(a, e.phys, e.zplan, e.stream); break;
-    }
-  } else if (e.kernel == 2) {
-    if (dt) launch_zmarch<T, MAG, ENT, true>(a, e.phys, e.zplan, e.stream);
-    else launch_zmarch<T, MAG, ENT, false>(a, e.phys, e.zplan, e.stream);
-  } else {
-    if (dt) launch_direct<T, MAG, ENT, true>(a, e.phys, e.stream);
-    else launch_direct<T, MAG, ENT, false>(a, e.phys, e.stream);
-  }
+long nblocks(const Engine& e) {
+  return e.kernel >= 2 ? e.zplan.nblocks : direct_blocks(e.par.nx, e.par.ny, e.par.nz);
 }
 
-template <typename T>
-void launch_p(const Engine& e, const KArgs& a, bool dt) {
-  if (e.mag) {
-    if (e.ent) launch_t<T, true, true>(e, a, dt);
-    else launch_t<T, true, false>(e, a, dt);
-  } else {
-    if (e.ent) launch_t<T, false, true>(e, a, dt);
-    else launch_t<T, false, false>(e, a, dt);
-  }
+LaunchCfg cfg_of(const Engine& e, bool dt) {
+  LaunchCfg c{};
+  c.kernel = e.kernel;
+  c.tile = e.tile;
+  c.mag = e.mag;
+  c.ent = e.ent;
+  c.dt = dt;
+  c.zp = e.zplan;
+  return c;
 }
-
-long nblocks(const Engine& e) { return e.kernel >= 2 ? e.zplan.nblocks : direct_blocks(e.par); }
 
 int launch_rhs(Engine& e, const KArgs& a, bool dt) {
-  if (e.par.precision == PCMX_PREC_MIXED) launch_p<float>(e, a, dt);
-  else launch_p<double>(e, a, dt);
+  const LaunchCfg c = cfg_of(e, dt);
+  const bool ok = (e.par.precision == PCMX_PREC_MIXED) ? launch_rhs_f32(c, a, e.phys, e.stream)
+                                                       : launch_rhs_f64(c, a, e.phys, e.stream);
+  if (!ok) return fail(PCMX_ERR_ARG, "no kernel variant for stage %d of kernel %d", a.stage, e.kernel);
   HIPCHECK(hipGetLastError());
   return PCMX_OK;
 }
@@ -351,13 +328,23 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   e->ent = par->iss >= 0;
   // kernel choice: the z-marching LDS kernels need whole tiles in x and y
   int kern = par->kernel;
-  if (kern == 0) kern = zmarch_supported(*par) ? 3 : 1;
-  if (kern >= 2 && !zmarch_supported(*par)) {
+  const int tile = (par->nz >= 4) ? split_tile(par->nx, par->ny) : -1;
+  if (kern == 0) kern = tile >= 0 ? 3 : 1;
+  if (kern == 2 && !zmarch_supported(*par)) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
   }
+  if (kern == 3 && tile < 0) {
+    delete e;
+    return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% 32 == 0 and ny %% 8 == 0, or nx and ny multiples of 16");
+  }
   e->kernel = kern;
-  if (kern >= 2) e->zplan = zmarch_plan(*par);
+  if (kern == 2) e->zplan = zmarch_plan(*par);
+  if (kern == 3) {
+    const char* env = std::getenv("PCMX_ZM_LZ");
+    e->tile = tile;
+    e->zplan = tile_plan(par->nx, par->ny, par->nz, tile_tx(tile), tile_ty(tile), env ? std::atoi(env) : 0);
+  }
   const size_t bytes = sizeof(double) * e->nxyz * e->nvar;
   auto alloc = [&](double** p, size_t n) -> int {
     hipError_t err = hipMalloc((void**)p, n);
@@ -365,7 +352,7 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
       return fail(PCMX_ERR_HIP, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(err));
     return PCMX_OK;
   };
-  e->partials_cap = std::max(direct_blocks(*par), e->kernel >= 2 ? e->zplan.nblocks : 0L);
+  e->partials_cap = std::max(direct_blocks(par->nx, par->ny, par->nz), e->kernel >= 2 ? e->zplan.nblocks : 0L);
   if ((rc = alloc(&e->f[0], bytes)) || (rc = alloc(&e->f[1], bytes)) || (rc = alloc(&e->w, bytes)) ||
       (rc = alloc(&e->partials, sizeof(double) * NRED * e->partials_cap)) ||
       (rc = alloc(&e->results, sizeof(double) * R_COUNT)) ||

@@ -1,0 +1,15 @@
+// pcmx_kern_f32.hip -- instantiations of every RHS kernel variant in float.
+#include "pcmx_dispatch.h"
+
+namespace pcmx {
+
+bool launch_rhs_f32(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  return launch_rhs_t<float>(c, a, P, s);
+}
+
+bool launch_ens_f32(const LaunchCfg& c, int stage, const EnsBox* boxes, const EnsWork* work, int nwork,
+                    double* partials, int nblocks, hipStream_t s) {
+  return launch_ens_t<float>(c, stage, boxes, work, nwork, partials, nblocks, s);
+}
+
+}  // namespace pcmx

@@ -41,34 +41,40 @@ inline bool zmarch_supported(const pcmx_params& p) {
   return p.nx % ZM_TX == 0 && p.ny % ZM_TY == 0 && p.nz >= 4 && (long)p.nx * p.ny < (1L << 31);
 }
 
-// Chooses the z-chunk length: the largest divisor of nz that still gives at
-// least `target` blocks (8 per CU), but not below 8 planes.  PCMX_ZM_LZ
-// overrides (for tuning sweeps).
-inline ZPlan zmarch_plan(const pcmx_params& p) {
+// Chooses the z-chunk length for tiles of tx x ty cells: the largest divisor
+// of nz that still gives at least `target` blocks (8 per CU), but not below 8
+// planes; lz_req > 0 (a divisor of nz) overrides.
+inline ZPlan tile_plan(int nx, int ny, int nz, int tx, int ty, int lz_req) {
   ZPlan z;
-  z.ntx = p.nx / ZM_TX;
-  z.nty = p.ny / ZM_TY;
+  z.ntx = nx / tx;
+  z.nty = ny / ty;
   const long ntiles = (long)z.ntx * z.nty;
   const long target = 2048;
-  int lz = p.nz;
-  const char* env = std::getenv("PCMX_ZM_LZ");
-  if (env && std::atoi(env) > 0 && p.nz % std::atoi(env) == 0) {
-    lz = std::atoi(env);
+  int lz = nz;
+  if (lz_req > 0 && nz % lz_req == 0) {
+    lz = lz_req;
   } else {
-    int best = p.nz;
-    for (int d = p.nz; d >= 1; --d) {
-      if (p.nz % d) continue;
-      if (d < 8 && d != p.nz) break;
+    int best = nz;
+    for (int d = nz; d >= 1; --d) {
+      if (nz % d) continue;
+      if (d < 8 && d != nz) break;
       best = d;
-      if (ntiles * (p.nz / d) >= target) break;
+      if (ntiles * (nz / d) >= target) break;
     }
     lz = best;
   }
   z.lz = lz;
-  z.nchunks = p.nz / lz;
+  z.nchunks = nz / lz;
   z.nblocks = ntiles * z.nchunks;
   z.xcd_remap = (z.nblocks % 8 == 0) ? 1 : 0;
   return z;
+}
+
+// The first z-marching kernel's plan (32 x 8 tiles); PCMX_ZM_LZ overrides the
+// chunk length (tuning sweeps).
+inline ZPlan zmarch_plan(const pcmx_params& p) {
+  const char* env = std::getenv("PCMX_ZM_LZ");
+  return tile_plan(p.nx, p.ny, p.nz, ZM_TX, ZM_TY, env ? std::atoi(env) : 0);
 }
 
 template <bool MAG, bool ENT>

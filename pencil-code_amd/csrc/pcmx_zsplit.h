@@ -36,6 +36,7 @@
 
 #include "pcmx_direct.h"
 #include "pcmx_kernels.h"
+#include "pcmx_launch.h"
 #include "pcmx_zmarch.h"
 
 namespace pcmx {
@@ -52,8 +53,6 @@ namespace pcmx {
 // separate update pass an explicit Courant step needs once dt is known:
 // 9 state accesses per variable and step plus the w1 halo, against 12.  With
 // a fixed step the explicit form is cheaper (w1 would be read with halo).
-enum Stage { ST_S1 = 0, ST_S2 = 1, ST_S3 = 2, ST_W = 3, ST_DT = 4, ST_S2D = 5 };
-
 template <int ST>
 struct StageT {
   static constexpr bool FORM1 = ST == ST_S2D;
@@ -64,43 +63,69 @@ struct StageT {
   static constexpr bool DIAG = ST == ST_S1 || ST == ST_W || ST == ST_DT;
 };
 
+// Tile geometry: a wave group of 256 threads owns TX x TY cells of a z
+// plane; the staged plane carries a 3-cell halo on each side.
+template <int TX_, int TY_>
+struct ZTile {
+  static constexpr int TX = TX_, TY = TY_, NT = TX_ * TY_, PX = TX_ + 6, PY = TY_ + 6, PLANE = PX * PY;
+  static_assert(NT == 256, "one thread per cell of the tile, 256 per wave group");
+};
+using ZT32x8 = ZTile<32, 8>;
+using ZT16x16 = ZTile<16, 16>;
+
 template <bool MAG>
 constexpr int zs_threads() {
-  return MAG ? 2 * ZM_NT : ZM_NT;
+  return MAG ? 512 : 256;
 }
 
-template <typename T, bool MAG, bool ENT, bool POLY, int ST>
-__global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
+// Per-variable stride of a staged plane in LDS, and the staging slots: K
+// slots per variable at elements tid + k*NTOT (a wave-uniform base plus a
+// 32-bit lane offset) and, when the remainders of all variables fit one pass
+// (PACK), one shared slot for them; otherwise the per-variable slots cover
+// the plane and the LDS plane is padded to K*NTOT so that no store needs a
+// guard.
+template <class TL, int NV, int NTOT>
+struct ZStage {
+  static constexpr int K0 = TL::PLANE / NTOT;
+  static constexpr int R = TL::PLANE - K0 * NTOT;
+  static constexpr bool PACK = R > 0 && K0 > 0 && NV * R <= NTOT;
+  static constexpr int K = PACK ? K0 : (TL::PLANE + NTOT - 1) / NTOT;
+  static constexpr int VS = PACK ? TL::PLANE : K * NTOT;
+};
+
+template <typename T, bool MAG, bool ENT, class TL>
+constexpr int zs_ring_elems() {
+  return ZM_RING * Slots<ENT, MAG>::NV * ZStage<TL, Slots<ENT, MAG>::NV, zs_threads<MAG>()>::VS;
+}
+constexpr int ZS_NXC = 5;  // exchange slots per cell: jxbr[3], j2, advec_va2
+
+// Marches one tile column (x0, y0, z0 .. z0+lz-1) of one box through the
+// RK stage ST.  ring and xch are the block's LDS buffers; the column's
+// reduction terms are added to red (all threads of the block take part).
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+__device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int x0, int y0, int z0, int lz,
+                                              T* ring, T* xch, double (&red)[NRED]) {
   using S = Slots<ENT, MAG>;
   using N = ZNeed<MAG, ENT>;
   using G = StageT<ST>;
   constexpr bool DT = G::DT;
   constexpr int NV = S::NV, IA = S::IAX, ISS = S::ISS;
   constexpr int NTOT = zs_threads<MAG>();
-  constexpr int SLOT = NV * ZM_PLANE;
+  using ZS = ZStage<TL, NV, NTOT>;
+  constexpr int VS = ZS::VS;
+  constexpr int SLOT = NV * VS;
   constexpr int NF = ENT ? 5 : 4;              // flow variables: uu, lnrho, [ss]
   constexpr int NGV = NF;                      // largest group (the flow half)
-  constexpr int NXC = 5;                       // exchange slots: jxbr[3], j2, advec_va2
   constexpr bool MIXED = sizeof(T) == 4;
-  __shared__ T ring[ZM_RING * SLOT];
-  __shared__ T xch[MAG ? NXC * ZM_NT : 1];
+  constexpr int PX = TL::PX;
 
   const int tid = threadIdx.x;
-  const int grp = MAG ? tid / ZM_NT : 0;  // wave-uniform
-  const int t = tid % ZM_NT;
-  const int tx = t % ZM_TX, ty = t / ZM_TX;
-  // XCD-aware work order: consecutive work items (neighbouring tiles that
-  // share halo rims) go to the same XCD, whose L2 then serves the rims.
-  const long b = blockIdx.x;
-  const long wid = zp.xcd_remap ? (b % 8) * (zp.nblocks / 8) + b / 8 : b;
-  const int txt = (int)(wid % zp.ntx);
-  const long rest = wid / zp.ntx;
-  const int tyt = (int)(rest % zp.nty);
-  const int chunk = (int)(rest / zp.nty);
-  const int x0 = txt * ZM_TX, y0 = tyt * ZM_TY, z0 = chunk * zp.lz;
+  const int grp = MAG ? tid / TL::NT : 0;  // wave-uniform
+  const int t = tid % TL::NT;
+  const int tx = t % TL::TX, ty = t / TL::TX;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long nxy = (long)nx * ny, nxyz = a.nxyz;
-  const int zlast = z0 + zp.lz - 1;
+  const int zlast = z0 + lz - 1;
 
   auto zoff = [&](int p) -> long {
     int zz = p % nz;
@@ -110,7 +135,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   auto slot_of = [&](int p) -> int { return (p - z0 + 3) & 3; };
   // offset inside a z plane of element ee of the tile's haloed plane
   auto plane_off = [&](int ee) -> unsigned {
-    const int row = ee / ZM_PX, col = ee % ZM_PX;
+    const int row = ee / PX, col = ee % PX;
     int gx = x0 - 3 + col, gy = y0 - 3 + row;
     gx += (gx < 0) ? nx : 0;
     gx -= (gx >= nx) ? nx : 0;
@@ -119,22 +144,21 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     return (unsigned)(gy * nx + gx);
   };
 
-  // Plane staging.  Each variable's haloed plane (ZM_PLANE values) is
-  // covered by K full slots, element tid + k*NTOT of every variable (so a
-  // load is a wave-uniform base, variable and plane, plus a 32-bit lane
-  // offset), and one remainder slot: the last R elements of every variable,
-  // NV*R of them, one each for threads tid < NV*R.
-  constexpr int K = ZM_PLANE / NTOT;
-  constexpr int R = ZM_PLANE - K * NTOT;
-  static_assert(NV * R <= NTOT, "remainder slot must fit in one pass");
+  // Plane staging (see ZStage): K per-variable slots plus, with PACK, the
+  // shared remainder slot.
+  constexpr int K = ZS::K;
+  constexpr int R = ZS::R;
   unsigned poff[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) poff[k] = plane_off(tid + k * NTOT);
-  const bool has_rem = tid < NV * R;
+  for (int k = 0; k < K; ++k) {
+    const int e = tid + k * NTOT;
+    poff[k] = plane_off(e < TL::PLANE ? e : TL::PLANE - 1);  // padding lanes load a valid duplicate
+  }
+  const bool has_rem = ZS::PACK && tid < NV * R;
   const int rv = has_rem ? tid / R : 0;
   const int re = K * NTOT + (has_rem ? tid % R : 0);
-  const unsigned roff = plane_off(re);
-  const int rl = rv * ZM_PLANE + re;
+  const unsigned roff = plane_off(re < TL::PLANE ? re : TL::PLANE - 1);
+  const int rl = rv * VS + re;
   const double rmu = MIXED ? a.means[rv] : 0.0;
 
   // dt*beta1 of the deferred first-substep update (stage S2)
@@ -174,7 +198,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const double val = staged(pre[v * K + k], prw[G::FORM1 ? v * K + k : 0]);
-        dst[v * ZM_PLANE + tid + k * NTOT] = MIXED ? (T)(val - mu) : (T)val;
+        dst[v * VS + tid + k * NTOT] = MIXED ? (T)(val - mu) : (T)val;
       }
     }
     if (has_rem) {
@@ -187,7 +211,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   auto var_of = [&](int g, int i) -> int { return g == 0 ? (i < 4 ? i : ISS) : IA + i; };
   const int x = x0 + tx, y = y0 + ty;
   const unsigned own = (unsigned)(y * nx + x);  // offset inside a z plane
-  const int c = (ty + 3) * ZM_PX + (tx + 3);
+  const int c = (ty + 3) * PX + (tx + 3);
   // start of variable v's z plane p in an array of the state layout (uniform)
   auto vplane = [&](auto* base, int v, int p) { return base + ((long)v * nxyz + (long)p * nxy); };
 
@@ -235,10 +259,6 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     cxz[i][0] = cxz[i][1] = cxz[i][2] = (T)0;
     cyz[i][0] = cyz[i][1] = cyz[i][2] = (T)0;
   }
-  double red[NRED];
-#pragma unroll
-  for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-
   const T fx1 = (T)P.fac1[0], fy1 = (T)P.fac1[1], fz1 = (T)P.fac1[2];
   const T fx2 = (T)P.fac2[0], fy2 = (T)P.fac2[1], fz2 = (T)P.fac2[2];
   const T C0 = (T)PCMX_D2C0, C1 = (T)PCMX_D2C1, C2 = (T)PCMX_D2C2, C3 = (T)PCMX_D2C3;
@@ -249,15 +269,15 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   auto derive = [&](auto emit_c, int v, int i, Derivs<T>& d, const T* L0, const T* L1, const T* L2,
                     const T* L3) {
     constexpr bool EMIT = decltype(emit_c)::value;
-    const T* p0 = L0 + v * ZM_PLANE;
+    const T* p0 = L0 + v * VS;
     const T c0 = p0[0];
     const T xp1 = p0[1], xm1 = p0[-1], xp2 = p0[2], xm2 = p0[-2], xp3 = p0[3], xm3 = p0[-3];
-    const T yp1 = p0[ZM_PX], ym1 = p0[-ZM_PX], yp2 = p0[2 * ZM_PX], ym2 = p0[-2 * ZM_PX];
-    const T yp3 = p0[3 * ZM_PX], ym3 = p0[-3 * ZM_PX];
+    const T yp1 = p0[PX], ym1 = p0[-PX], yp2 = p0[2 * PX], ym2 = p0[-2 * PX];
+    const T yp3 = p0[3 * PX], ym3 = p0[-3 * PX];
     if constexpr (EMIT) {
-      const T* p1 = L1 + v * ZM_PLANE;
-      const T* p2 = L2 + v * ZM_PLANE;
-      const T* p3 = L3 + v * ZM_PLANE;
+      const T* p1 = L1 + v * VS;
+      const T* p2 = L2 + v * VS;
+      const T* p3 = L3 + v * VS;
       const T zp1 = p1[0], zp2 = p2[0], zp3 = p3[0];
       const T zm1 = back[i][0], zm2 = back[i][1], zm3 = back[i][2];
       // der_main (deriv.f90:124-162)
@@ -272,9 +292,9 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
       }
       T mxy = 0, mxzv = 0, myzv = 0;
       if (N::xy(v)) {  // bidiagonal xy, operand order of deriv.f90:1410-1418
-        const T b1 = p0[1 + ZM_PX] - p0[-1 + ZM_PX] + p0[-1 - ZM_PX] - p0[1 - ZM_PX];
-        const T b2 = p0[2 + 2 * ZM_PX] - p0[-2 + 2 * ZM_PX] + p0[-2 - 2 * ZM_PX] - p0[2 - 2 * ZM_PX];
-        const T b3 = p0[3 + 3 * ZM_PX] - p0[-3 + 3 * ZM_PX] + p0[-3 - 3 * ZM_PX] - p0[3 - 3 * ZM_PX];
+        const T b1 = p0[1 + PX] - p0[-1 + PX] + p0[-1 - PX] - p0[1 - PX];
+        const T b2 = p0[2 + 2 * PX] - p0[-2 + 2 * PX] + p0[-2 - 2 * PX] - p0[2 - 2 * PX];
+        const T b3 = p0[3 + 3 * PX] - p0[-3 + 3 * PX] + p0[-3 - 3 * PX] - p0[3 - 3 * PX];
         mxy = (T)P.facxy * ((T)270.0 * b1 - (T)27.0 * b2 + (T)2.0 * b3);
       }
       if (N::xz(v)) {
@@ -282,8 +302,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         mxzv = (T)P.facxz * (plus - cxz[N::ixz(v) & 1][0]);
       }
       if (N::yz(v)) {
-        const T plus = (T)270.0 * (p1[ZM_PX] - p1[-ZM_PX]) - (T)27.0 * (p2[2 * ZM_PX] - p2[-2 * ZM_PX]) +
-                       (T)2.0 * (p3[3 * ZM_PX] - p3[-3 * ZM_PX]);
+        const T plus = (T)270.0 * (p1[PX] - p1[-PX]) - (T)27.0 * (p2[2 * PX] - p2[-2 * PX]) +
+                       (T)2.0 * (p3[3 * PX] - p3[-3 * PX]);
         myzv = (T)P.facyz * (plus - cyz[N::iyz(v) & 1][0]);
       }
       file_derivs<T, MAG, ENT>(d, v, dx, dy, dz, dxx, dyy, dzz, mxy, mxzv, myzv);
@@ -376,8 +396,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
           lnrho = (T)fcur[NGV + 3];
         } else {
 #pragma unroll
-          for (int j = 0; j < 3; ++j) d.u[j] = L0[j * ZM_PLANE];
-          lnrho = L0[3 * ZM_PLANE];
+          for (int j = 0; j < 3; ++j) d.u[j] = L0[j * VS];
+          lnrho = L0[3 * VS];
         }
         const T rho1 = pexp<T>(-lnrho);
         double wa[3];
@@ -387,9 +407,9 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         double b2 = 0.0;
         field_point<T, ENT, DT>(d, rho1, P, wa, fo, &b2);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) xch[j * ZM_NT + t] = fo.jxbr[j];
-        if constexpr (ENT) xch[3 * ZM_NT + t] = fo.j2;
-        if constexpr (DT) xch[4 * ZM_NT + t] = fo.advec_va2;
+        for (int j = 0; j < 3; ++j) xch[j * TL::NT + t] = fo.jxbr[j];
+        if constexpr (ENT) xch[3 * TL::NT + t] = fo.j2;
+        if constexpr (DT) xch[4 * TL::NT + t] = fo.advec_va2;
         if constexpr (G::UPDATE) {
 #pragma unroll
           for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(fa[i] + dtb * wa[i], &vplane(a.fout, IA + i, q)[own]);
@@ -413,9 +433,9 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
         FieldOut<T> fo{};
         if constexpr (MAG) {
 #pragma unroll
-          for (int j = 0; j < 3; ++j) fo.jxbr[j] = xch[j * ZM_NT + t];
-          if constexpr (ENT) fo.j2 = xch[3 * ZM_NT + t];
-          if constexpr (DT) fo.advec_va2 = xch[4 * ZM_NT + t];
+          for (int j = 0; j < 3; ++j) fo.jxbr[j] = xch[j * TL::NT + t];
+          if constexpr (ENT) fo.j2 = xch[3 * TL::NT + t];
+          if constexpr (DT) fo.advec_va2 = xch[4 * TL::NT + t];
         }
         double dt1 = 0.0;
         flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
@@ -464,19 +484,82 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
     run(std::integral_constant<int, 0>{});
   }
 
-  if constexpr (G::DIAG) {
-    if (a.partials) block_reduce_store<NTOT>(red, a.partials, (long)blockIdx.x);
+}
+
+// One box: block b marches tile column wid(b) (XCD-aware: consecutive work
+// items, neighbouring tiles that share halo rims, go to the same XCD, whose
+// L2 then serves the rims).
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+__global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
+  __shared__ T ring[zs_ring_elems<T, MAG, ENT, TL>()];
+  __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
+  const long b = blockIdx.x;
+  const long wid = zp.xcd_remap ? (b % 8) * (zp.nblocks / 8) + b / 8 : b;
+  const int txt = (int)(wid % zp.ntx);
+  const long rest = wid / zp.ntx;
+  const int tyt = (int)(rest % zp.nty);
+  const int chunk = (int)(rest / zp.nty);
+  double red[NRED];
+#pragma unroll
+  for (int q = 0; q < NRED; ++q) red[q] = 0.0;
+  zsplit_column<T, MAG, ENT, POLY, ST, TL>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring, xch, red);
+  if constexpr (StageT<ST>::DIAG) {
+    if (a.partials) block_reduce_store<zs_threads<MAG>()>(red, a.partials, (long)blockIdx.x);
   }
 }
 
-template <typename T, bool MAG, bool ENT, int ST>
+// Ensemble of boxes: per-box launch arguments and one tile column per work
+// item, all in device memory.
+struct EnsBox {
+  KArgs a;  // this box's arrays, sizes and RK coefficients for the launch
+  Phys P;   // this box's constants
+};
+struct EnsWork {
+  int box, x0, y0, z0, lz;
+};
+
+// Persistent launch over the work items of every box: the blocks of XCD x
+// take the x-th contiguous eighth of the work list in turn (items of one box
+// are contiguous, neighbouring tiles first), so tiles that share halo rims
+// run on one XCD.  partials[w] receives work item w's reduction terms.  Every
+// block passes the same number of barriers per item, and the loop bounds are
+// uniform, so all waves of a block leave the loop together.
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+__global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsBox* boxes, const EnsWork* work,
+                                                                      int nwork, double* partials) {
+  __shared__ T ring[zs_ring_elems<T, MAG, ENT, TL>()];
+  __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
+  const int nb = gridDim.x;
+  // with a block count divisible by 8 every XCD holds nb/8 blocks: XCD x then
+  // takes the x-th eighth of the list; otherwise plain striding
+  const bool by_xcd = (nb % 8) == 0;
+  const int xcd = by_xcd ? blockIdx.x % 8 : 0, rank = by_xcd ? blockIdx.x / 8 : blockIdx.x;
+  const int stride = by_xcd ? nb / 8 : nb;
+  const int w0 = by_xcd ? (int)((long)nwork * xcd / 8) : 0;
+  const int w1 = by_xcd ? (int)((long)nwork * (xcd + 1) / 8) : nwork;
+  for (int w = w0 + rank; w < w1; w += stride) {
+    const EnsWork it = work[w];
+    const KArgs a = boxes[it.box].a;
+    const Phys P = boxes[it.box].P;
+    double red[NRED];
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) red[q] = 0.0;
+    zsplit_column<T, MAG, ENT, POLY, ST, TL>(a, P, it.x0, it.y0, it.z0, it.lz, ring, xch, red);
+    if constexpr (StageT<ST>::DIAG) {
+      if (partials) block_reduce_store<zs_threads<MAG>()>(red, partials, (long)w);
+    }
+    __syncthreads();  // the next item reuses the ring
+  }
+}
+
+template <typename T, bool MAG, bool ENT, int ST, class TL>
 void launch_zsplit(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
   if (!ENT && P.polytropic)
-    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST>), dim3((unsigned)zp.nblocks), dim3(zs_threads<MAG>()),
-                       0, s, a, P, zp);
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST, TL>), dim3((unsigned)zp.nblocks),
+                       dim3(zs_threads<MAG>()), 0, s, a, P, zp);
   else
-    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST>), dim3((unsigned)zp.nblocks), dim3(zs_threads<MAG>()),
-                       0, s, a, P, zp);
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST, TL>), dim3((unsigned)zp.nblocks),
+                       dim3(zs_threads<MAG>()), 0, s, a, P, zp);
 }
 
 }  // namespace pcmx

@@ -51,6 +51,18 @@ class Params(ctypes.Structure):
     ]
 
 
+class Box(ctypes.Structure):
+    """One box of an ensemble (pcmx_box)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("dt", ctypes.c_double),
+    ]
+
+
+ENS_NO_GRAPH = 1
+
+
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
                 cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
                 kernel=None, L=2 * math.pi):
@@ -102,6 +114,15 @@ def lib():
             "pcmx_time_step": [ctypes.c_int, dp, dp, ctypes.c_int],
             "pcmx_kernel_timing": [dp],
             "pcmx_set_timing": [ctypes.c_int],
+            "pcmx_ens_create": [ctypes.POINTER(Params), ctypes.POINTER(Box), ctypes.c_int, ctypes.c_int],
+            "pcmx_ens_destroy": [],
+            "pcmx_ens_load": [ctypes.c_int, dp],
+            "pcmx_ens_copy": [ctypes.c_int, dp],
+            "pcmx_ens_step": [ctypes.c_int, ctypes.c_int],
+            "pcmx_ens_reduced": [dp],
+            "pcmx_ens_tune": [ctypes.c_int],
+            "pcmx_ens_set_timing": [ctypes.c_int],
+            "pcmx_ens_timing": [dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -251,3 +272,56 @@ def advance(interior, par, nsteps, dt, t=0.0, per_substep=False):
         return interior_of(out, par.mvar).copy(), t, dt
     finally:
         finalize_gpu()
+
+
+# ------------------------------------------------------------------ ensembles
+def make_box(n, nu, eta, dt, L=2 * math.pi):
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+    Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    return Box(nx, ny, nz, Lx / nx, Ly / ny, Lz / nz, nu, eta, dt)
+
+
+def ens_create(par, boxes, flags=0):
+    """pcmx_ens_create: boxes is a list of Box; par gives the shared physics."""
+    arr = (Box * len(boxes))(*boxes)
+    _check(lib().pcmx_ens_create(ctypes.byref(par), arr, len(boxes), flags))
+
+
+def ens_destroy():
+    _check(lib().pcmx_ens_destroy())
+
+
+def ens_load(ibox, interior):
+    a = np.ascontiguousarray(interior, dtype=np.float64)
+    _check(lib().pcmx_ens_load(ibox, _ptr(a)))
+
+
+def ens_copy(ibox, shape):
+    out = np.zeros(shape)
+    _check(lib().pcmx_ens_copy(ibox, _ptr(out)))
+    return out
+
+
+def ens_step(nsteps, ldiag=True):
+    _check(lib().pcmx_ens_step(nsteps, int(ldiag)))
+
+
+def ens_reduced(nbox):
+    out = np.zeros(3 * nbox)
+    _check(lib().pcmx_ens_reduced(_ptr(out)))
+    return out.reshape(nbox, 3)
+
+
+def ens_tune(nsteps_probe=2):
+    _check(lib().pcmx_ens_tune(nsteps_probe))
+
+
+def ens_set_timing(enable):
+    _check(lib().pcmx_ens_set_timing(int(enable)))
+
+
+def ens_timing():
+    """(mean ms per substep, substeps, tile width, z-chunk limit, blocks per CU, work items)."""
+    out = np.zeros(6)
+    _check(lib().pcmx_ens_timing(_ptr(out)))
+    return float(out[0]), int(out[1]), int(out[2]), int(out[3]), int(out[4]), int(out[5])

@@ -62,3 +62,19 @@ def test_initialize_rejects_bad_parameters(mutate, msg):
 def test_entropy_requires_gamma_not_one():
     p = pcmx.make_params(16, lentropy=True, gamma=1.0)
     assert pcmx.lib().pcmx_initialize(None, 0, ctypes.byref(p), 0.0, 0) == -1
+
+
+def test_ensemble_calls_before_create_fail_with_state_error():
+    lib = pcmx.lib()
+    assert lib.pcmx_ens_step(1, 0) == -2
+    assert b"pcmx_ens_create" in lib.pcmx_last_error()
+
+
+@pytest.mark.parametrize("bad", ["untiled", "entropy", "courant", "no_dt"])
+def test_ensemble_rejects_unsupported_setups_before_allocating(bad):
+    par = pcmx.make_params(32, lmagnetic=True, lentropy=(bad == "entropy"), gamma=5 / 3 if bad == "entropy" else 1.0,
+                           ldt=(bad == "courant"), eta=1e-3)
+    boxes = [pcmx.make_box(32, 1e-3, 1e-3, 1e-3), pcmx.make_box(24 if bad == "untiled" else 48, 1e-3, 1e-3,
+                                                                0.0 if bad == "no_dt" else 1e-3)]
+    with pytest.raises(pcmx.PcmxError):
+        pcmx.ens_create(par, boxes)
