@@ -61,11 +61,18 @@ def rel_linf(a, b):
     return out
 
 
-def run_form(f, par, steps, warmup, dt, keep_state=False):
-    """warmup + timed steps of one engine; returns timings and, optionally,
-    the final interior state."""
+def run_form(f, par, steps, warmup, dt, keep_state=False, tune=True):
+    """warmup + timed steps of one engine (tile shape and z-chunk length tuned
+    first when the split kernel runs); returns timings and, optionally, the
+    final interior state."""
     pcmx.initialize_gpu(f, par, nt=1)
+    plan = None
     try:
+        if tune:
+            try:
+                plan = pcmx.tune()
+            except pcmx.PcmxError:
+                plan = None  # not the split kernel (grid not tileable)
         t = 0.0
         if warmup:
             t, dt = pcmx.time_step(warmup, t, dt, ldiag=True)
@@ -87,7 +94,7 @@ def run_form(f, par, steps, warmup, dt, keep_state=False):
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     return dict(value=ncell * steps / sec / 1e6, sec=sec, ms_per_step=sec * 1e3 / steps, kernel_ms=kms,
                 nkernels=nk, achieved=achieved, urms=float(red[0]), brms=float(red[2]), t=t, dt=dt,
-                state=state)
+                state=state, plan=(f"{plan[1]}x{256 // plan[1]} tiles, {plan[2]}-plane z chunks" if plan else None))
 
 
 def measured_traffic(n):
@@ -175,7 +182,7 @@ def run_config(k, threads):
         r = run_form(f, par, c["steps"], 0, c["dt"], keep_state=True)
         res[name] = r
         out[name] = {"value": round(r["value"], 2), "kernel_ms": round(r["kernel_ms"], 4),
-                     "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "t": r["t"]}
+                     "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "t": r["t"], "plan": r["plan"]}
         log(f"bench: config {k} {name}: {out[name]}")
     out["mixed_vs_fp64_rel_linf"] = rel_linf(res["mixed"]["state"], res["fp64"]["state"])
     if c["oracle_steps"]:
@@ -343,6 +350,7 @@ def main():
                                f"fixed dt={args.dt}, band-limited uu/aa (1<=|k|<=4, rms 0.05/0.02), "
                                "lnrho=0, seed 7",
                    "grid": n, "nvar": 7, "parallelism": "single GPU"},
+        "plan": r64["plan"],
         "roofline": {"bound": "hbm", "achieved": round(r64["achieved"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r64["achieved"] / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(n),

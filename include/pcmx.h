@@ -137,6 +137,13 @@ int pcmx_split_update(void);
  * With ldiag, urms/umax/brms are reduced on every step's first substep. */
 int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag);
 
+/* Times the tile shapes and z-chunk lengths the grid allows for the split
+ * kernel (kernel 3) on the current state, without changing it, and keeps the
+ * fastest.  best[0] = ms per launch of the chosen plan, best[1] = tile width,
+ * best[2] = z-chunk length (best may be NULL).  No counterpart in the
+ * reference (its GPU library fixes its launch shapes at build time). */
+int pcmx_tune(double* best);
+
 /* Launch statistics of the last pcmx_time_step: averages in milliseconds of
  * the fused RHS kernel per substep, measured with HIP events on the library
  * stream.  out[0] = mean kernel ms, out[1] = number of kernels timed. */

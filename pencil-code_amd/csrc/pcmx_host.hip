@@ -599,6 +599,69 @@ int pcmx_test_bcs(double* maxdiff) {
   return PCMX_OK;
 }
 
+int pcmx_tune(double* best_ms) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (e.kernel != 3) return fail(PCMX_ERR_ARG, "pcmx_tune tunes the split z-marching kernel (kernel 3) only");
+  // candidates: every tile shape the grid allows, chunk lengths dividing nz
+  std::vector<std::pair<int, int>> cands;
+  for (int tile : {TILE_32x8, TILE_16x16}) {
+    if (e.par.nx % tile_tx(tile) || e.par.ny % tile_ty(tile)) continue;
+    for (int lz : {8, 16, 32, 64, 128, 256, 512, 1024})
+      if (lz <= e.par.nz && e.par.nz % lz == 0) cands.push_back({tile, lz});
+    if (e.par.nz > 1024 || e.par.nz % 8) cands.push_back({tile, e.par.nz});
+  }
+  // time the last-substep stage: it reads f[cur] and w and writes only the
+  // spare buffer, so the state is untouched
+  if (e.pending_f1) return fail(PCMX_ERR_STATE, "pcmx_tune between substeps 1 and 2 of a step");
+  KArgs a = base_args(e);
+  a.fin = e.f[e.cur];
+  a.fout = e.f[1 - e.cur];
+  a.alpha = ALPHA_TS[2];
+  a.beta = BETA_TS[2];
+  a.dtb = 0.0;
+  a.stage = ST_S3;
+  a.mode = MODE_UPDATE;
+  const int tile0 = e.tile;
+  const ZPlan zp0 = e.zplan;
+  hipEvent_t t0, t1;
+  HIPCHECK(hipEventCreate(&t0));
+  HIPCHECK(hipEventCreate(&t1));
+  float best = 1e30f;
+  int btile = tile0;
+  ZPlan bplan = zp0;
+  for (const auto& c : cands) {
+    e.tile = c.first;
+    e.zplan = tile_plan(e.par.nx, e.par.ny, e.par.nz, tile_tx(c.first), tile_ty(c.first), c.second);
+    if (e.zplan.nblocks > e.partials_cap) continue;
+    if ((rc = launch_rhs(e, a, false))) break;  // warm-up
+    HIPCHECK(hipEventRecord(t0, e.stream));
+    for (int it = 0; it < 2 && !rc; ++it) rc = launch_rhs(e, a, false);
+    if (rc) break;
+    HIPCHECK(hipEventRecord(t1, e.stream));
+    HIPCHECK(hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+    if (ms < best) {
+      best = ms;
+      btile = c.first;
+      bplan = e.zplan;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  e.tile = rc ? tile0 : btile;
+  e.zplan = rc ? zp0 : bplan;
+  if (rc) return rc;
+  if (best_ms) {
+    best_ms[0] = best / 2.0;
+    best_ms[1] = tile_tx(btile);
+    best_ms[2] = bplan.lz;
+  }
+  return PCMX_OK;
+}
+
 int pcmx_set_timing(int enable) {
   int rc = require();
   if (rc) return rc;

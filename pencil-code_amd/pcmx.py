@@ -113,6 +113,7 @@ def lib():
             "pcmx_split_update": [],
             "pcmx_time_step": [ctypes.c_int, dp, dp, ctypes.c_int],
             "pcmx_kernel_timing": [dp],
+            "pcmx_tune": [dp],
             "pcmx_set_timing": [ctypes.c_int],
             "pcmx_ens_create": [ctypes.POINTER(Params), ctypes.POINTER(Box), ctypes.c_int, ctypes.c_int],
             "pcmx_ens_destroy": [],
@@ -221,6 +222,14 @@ def time_step(nsteps, t, dt, ldiag=True):
     tt, dd = ctypes.c_double(t), ctypes.c_double(dt)
     _check(lib().pcmx_time_step(nsteps, ctypes.byref(tt), ctypes.byref(dd), int(ldiag)))
     return tt.value, dd.value
+
+
+def tune():
+    """pcmx_tune: pick the fastest tile shape and z-chunk length for the
+    loaded state.  Returns (ms per launch, tile width, z-chunk length)."""
+    out = np.zeros(3)
+    _check(lib().pcmx_tune(_ptr(out)))
+    return float(out[0]), int(out[1]), int(out[2])
 
 
 def set_timing(enable):

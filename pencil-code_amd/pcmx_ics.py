@@ -38,7 +38,7 @@ def bandlimited(ncomp, n, kmin, kmax, rms, seed):
     for c in range(ncomp):
         m = idx[0].size
         spec[idx] = rng.standard_normal(m) + 1j * rng.standard_normal(m)
-        out[c] = np.fft.irfftn(spec, s=(nz, ny, nx))
+        out[c] = np.fft.irfftn(spec, s=(nz, ny, nx), axes=(0, 1, 2))
         spec[idx] = 0
     cur = np.sqrt(np.mean(np.sum(out * out, axis=0)))
     return out * (rms / cur) if cur > 0 else out

@@ -188,3 +188,23 @@ def test_multi_round_grid(name):
     err1 = rel_linf(h1, o1)
     announce(f"  kernel 3 vs oracle, 1 step: {err1}")
     assert np.all(err1 <= 1e-12), err1
+
+
+@pytest.mark.parametrize("name", ["hydro_z", "mhd_z", "mhdss_z"])
+def test_tuned_plan_is_bitwise_identical(name):
+    """pcmx_tune only changes the tile shape and z-chunk length, which do not
+    change any cell's arithmetic, so results are bitwise equal."""
+    c, gp, op, f0 = setup(name, 3)
+    announce(f"test_tuned_plan_is_bitwise_identical[{name}]")
+    ref, _, _ = pcmx.advance(f0, gp, 3, c["dt"])
+    f = pcmx.farray_from_interior(f0)
+    pcmx.initialize_gpu(f, gp, nt=1)
+    try:
+        ms, tile, lz = pcmx.tune()
+        announce(f"  tuned: {ms:.4f} ms per launch, tile width {tile}, {lz}-plane chunks")
+        pcmx.time_step(3, 0.0, c["dt"])
+        out = np.zeros_like(f)
+        pcmx.copy_farray_from_gpu(out)
+    finally:
+        pcmx.finalize_gpu()
+    np.testing.assert_array_equal(pcmx.interior_of(out, gp.mvar), ref)
