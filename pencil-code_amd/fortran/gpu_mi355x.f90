@@ -32,7 +32,7 @@ module GPU
   logical :: ltest_rhs=.false.    !PAR-DOC: run the reference's test_rhs_gpu comparison (equ.f90)
   integer :: it_test_rhs=1
   integer :: gpu_precision=0      !PAR-DOC: 0 FP64, 1 mixed (FP32 stencils on box-mean offsets)
-  integer :: gpu_kernel=0         !PAR-DOC: 0 auto, 1 direct, 2 z-marching LDS kernel
+  integer :: gpu_kernel=0         !PAR-DOC: 0 auto, 1 direct, 2 z-marching LDS kernel, 3 split z-marching kernel
 !
   namelist /gpu_run_pars/ ltest_bcs, ltest_rhs, it_test_rhs, gpu_precision, gpu_kernel
 !
