@@ -78,31 +78,111 @@ constexpr int zs_threads() {
   return MAG ? 512 : 256;
 }
 
-// Per-variable stride of a staged plane in LDS, and the staging slots: K
-// slots per variable at elements tid + k*NTOT (a wave-uniform base plus a
-// 32-bit lane offset) and, when the remainders of all variables fit one pass
-// (PACK), one shared slot for them; otherwise the per-variable slots cover
-// the plane and the LDS plane is padded to K*NTOT so that no store needs a
-// guard.
-template <class TL, int NV, int NTOT>
+// Cells per staged plane in LDS (VS), and the staging slots: K slots per
+// staged plane at cells tid + k*NTOT (a wave-uniform base plus a 32-bit lane
+// offset) and, when the remainders fit (PACK), one more pass for them: with
+// pair planes, wave 0 stages the remainders of the NP pair planes and wave 1
+// those of the NS single planes (so the pair/single choice is uniform per
+// wave); with single planes only, threads tid < NS*R.  Otherwise the slots
+// cover the plane and the LDS plane is padded to K*NTOT so that no store
+// needs a guard.
+template <class TL, int NP, int NS, int NTOT>
 struct ZStage {
   static constexpr int K0 = TL::PLANE / NTOT;
   static constexpr int R = TL::PLANE - K0 * NTOT;
-  static constexpr bool PACK = R > 0 && K0 > 0 && NV * R <= NTOT;
+  static constexpr bool PACK = R > 0 && K0 > 0 &&
+                               (NP == 0 ? NS * R <= NTOT : NP * R <= 64 && NS * R <= 64 && NTOT >= 128);
   static constexpr int K = PACK ? K0 : (TL::PLANE + NTOT - 1) / NTOT;
   static constexpr int VS = PACK ? TL::PLANE : K * NTOT;
 };
 
-template <typename T, bool MAG, bool ENT, class TL>
-constexpr int zs_ring_elems() {
-  return ZM_RING * Slots<ENT, MAG>::NV * ZStage<TL, Slots<ENT, MAG>::NV, zs_threads<MAG>()>::VS;
+// LDS layout of a staged plane: NP pair planes, two variables interleaved
+// cell by cell, then NS single planes (PAIRS), or one single plane per
+// variable (!PAIRS).  One 128-bit LDS read (FP64) fetches
+// one stencil tap of both variables of a pair: on gfx950 such reads move
+// about 210 bytes per clock and CU, 64-bit reads of consecutive doubles
+// about 120, and 64-bit reads of one half of each pair (a 16-byte stride)
+// about 60 with bank conflicts.  So the variables without a partner get
+// planes of their own, and a tap that only one component of a pair needs is
+// still read at full width.  Pairs: (uu_x, uu_y), (uu_z, lnrho), [(aa_x,
+// aa_y)]; singles: [aa_z], [ss].  The pair layout needs a few more registers
+// (four per tap in flight instead of two); variants that would spill with it
+// use single planes (zs_pairs).
+template <class T>
+struct alignas(2 * sizeof(T)) Pair {
+  T x, y;
+};
+
+template <int C, class T>
+__device__ __forceinline__ T pick(const Pair<T>& e) {
+  if constexpr (C == 0) return e.x;
+  else return e.y;
 }
+template <int C, class T>
+__device__ __forceinline__ T pick(const T& e) {
+  return e;
+}
+
+template <bool ENT, bool MAG, bool PAIRS>
+struct ZLayout {
+  using S = Slots<ENT, MAG>;
+  static constexpr int NP = PAIRS ? (MAG ? 3 : 2) : 0;                 // pair planes
+  static constexpr int NS = PAIRS ? (MAG ? 1 : 0) + (ENT ? 1 : 0) : S::NV;  // single planes
+  static constexpr int NPL = NP + NS;                                  // staged planes
+  static constexpr int WIDTH = 2 * NP + NS;                            // elements per cell
+  static_assert(WIDTH == S::NV, "every variable staged once");
+  // single planes of aa_z (MAG) and ss (ENT)
+  static constexpr int SAZ = PAIRS ? 0 : S::IAX + 2;
+  static constexpr int SSS = PAIRS ? (MAG ? 1 : 0) : S::ISS;
+  // the variables of staged plane pl: components 0 and 1 of pair pl, or
+  // (twice) the variable of single plane pl - NP
+  __host__ __device__ static constexpr int v0(int pl) {
+    if (!PAIRS) return pl;
+    return pl == 0 ? 0 : pl == 1 ? 2 : pl < NP ? S::IAX : (MAG && pl == NP + SAZ ? S::IAX + 2 : S::ISS);
+  }
+  __host__ __device__ static constexpr int v1(int pl) {
+    if (!PAIRS) return pl;
+    return pl == 0 ? 1 : pl == 1 ? 3 : pl < NP ? S::IAX + 1 : v0(pl);
+  }
+};
+
+// Layout choice per kernel variant (ENS: the ensemble kernel): pair planes
+// unless the variant is known to run out of registers with them.  The
+// deferred first-substep stage holds the state and the 2N register of the
+// next plane in flight, and the ensemble kernel keeps each box's constants
+// in registers; both spill with pairs in FP64.
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool ENS>
+constexpr bool zs_pairs() {
+  if constexpr (sizeof(T) == 4) return true;
+  else if constexpr (ENS || ST == ST_S2D) return false;
+  else if constexpr (std::is_same_v<TL, ZTile<16, 16>>)
+    return !(ST == ST_S3 && !ENT && !POLY) && !(POLY && ST == ST_S2);
+  else return !POLY && !(ENT && ST == ST_W);
+}
+
+template <int I>
+using IC = std::integral_constant<int, I>;
+
+// Ties v to sink without an instruction: the value counts as used, so the
+// load that produced it keeps its full width.
+template <class T>
+__device__ __forceinline__ void hold_reg(T& sink, T v) {
+  asm("" : "+v"(sink) : "v"(v));
+}
+
+// Elements (of the state type) of the LDS ring.
+template <typename T, bool MAG, bool ENT, class TL, bool PAIRS>
+constexpr int zs_ring_elems() {
+  using LY = ZLayout<ENT, MAG, PAIRS>;
+  return ZM_RING * LY::WIDTH * ZStage<TL, LY::NP, LY::NS, zs_threads<MAG>()>::VS;
+}
+
 constexpr int ZS_NXC = 5;  // exchange slots per cell: jxbr[3], j2, advec_va2
 
 // Marches one tile column (x0, y0, z0 .. z0+lz-1) of one box through the
 // RK stage ST.  ring and xch are the block's LDS buffers; the column's
 // reduction terms are added to red (all threads of the block take part).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS>
 __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int x0, int y0, int z0, int lz,
                                               T* ring, T* xch, double (&red)[NRED]) {
   using S = Slots<ENT, MAG>;
@@ -111,9 +191,16 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   constexpr bool DT = G::DT;
   constexpr int NV = S::NV, IA = S::IAX, ISS = S::ISS;
   constexpr int NTOT = zs_threads<MAG>();
-  using ZS = ZStage<TL, NV, NTOT>;
-  constexpr int VS = ZS::VS;
-  constexpr int SLOT = NV * VS;
+  using LY = ZLayout<ENT, MAG, PAIRS>;
+  using T2 = Pair<T>;
+  constexpr int NP = LY::NP, NPL = LY::NPL;
+  using ZS = ZStage<TL, NP, LY::NS, NTOT>;
+  constexpr int VS = ZS::VS;               // cells per staged plane (padded)
+  constexpr int SLOT = LY::WIDTH * VS;     // elements per ring slot
+  static_assert(VS % 2 == 0, "pair planes stay 16-byte aligned");
+  // element offset in a ring slot of pair plane j / single plane s
+  auto pair_at = [](const T* slot, int j) { return reinterpret_cast<const T2*>(slot + 2 * j * VS); };
+  auto single_at = [](const T* slot, int sp) { return slot + (2 * NP + sp) * VS; };
   constexpr int NF = ENT ? 5 : 4;              // flow variables: uu, lnrho, [ss]
   constexpr int NGV = NF;                      // largest group (the flow half)
   constexpr bool MIXED = sizeof(T) == 4;
@@ -154,17 +241,23 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     const int e = tid + k * NTOT;
     poff[k] = plane_off(e < TL::PLANE ? e : TL::PLANE - 1);  // padding lanes load a valid duplicate
   }
-  const bool has_rem = ZS::PACK && tid < NV * R;
-  const int rv = has_rem ? tid / R : 0;
-  const int re = K * NTOT + (has_rem ? tid % R : 0);
+  // with pair planes, wave 0 stages pair remainders and wave 1 single ones
+  const bool rem_pair = NP > 0 && tid < 64;
+  const int rt = NP == 0 || rem_pair ? tid : tid - 64;
+  const bool has_rem = ZS::PACK && (rem_pair ? rt < NP * R : rt < LY::NS * R);
+  const int rp = has_rem ? (rem_pair ? 0 : NP) + rt / R : 0;  // staged plane of the remainder element
+  const int rva = LY::v0(rp), rvb = LY::v1(rp);
+  const int re = K * NTOT + (has_rem ? rt % R : 0);
   const unsigned roff = plane_off(re < TL::PLANE ? re : TL::PLANE - 1);
-  const int rl = rv * VS + re;
-  const double rmu = MIXED ? a.means[rv] : 0.0;
+  // remainder element offset in a slot: in pairs for a pair plane, in
+  // elements for a single plane
+  const int rl = rp < NP ? rp * VS + re : (2 * NP + rp - NP) * VS + re;
+  const double rmua = MIXED ? a.means[rva] : 0.0, rmub = MIXED ? a.means[rvb] : 0.0;
 
   // dt*beta1 of the deferred first-substep update (stage S2)
   const double dtb1 = a.d_dt ? a.d_dt[0] * a.beta1 : a.dtb1;
   constexpr int NW = G::FORM1 ? NV * K : 1;
-  double pre[NV * K], prem = 0.0, prw[NW], prwm = 0.0;
+  double pre[NV * K], prem[2] = {0.0, 0.0}, prw[NW], prwm[2] = {0.0, 0.0};
   auto load_plane = [&](int p) {
     const double* src = a.fin + zoff(p);
 #pragma unroll
@@ -173,7 +266,10 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 #pragma unroll
       for (int k = 0; k < K; ++k) pre[v * K + k] = sv[poff[k]];
     }
-    if (has_rem) prem = src[rv * nxyz + roff];
+    if (has_rem) {
+      prem[0] = src[rva * nxyz + roff];
+      prem[1] = src[rvb * nxyz + roff];
+    }
     if constexpr (G::FORM1) {
       const double* srw = a.w + zoff(p);
 #pragma unroll
@@ -182,7 +278,10 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 #pragma unroll
         for (int k = 0; k < K; ++k) prw[v * K + k] = sv[poff[k]];
       }
-      if (has_rem) prwm = srw[rv * nxyz + roff];
+      if (has_rem) {
+        prwm[0] = srw[rva * nxyz + roff];
+        prwm[1] = srw[rvb * nxyz + roff];
+      }
     }
   };
   // the staged value: f (or f1 = f0 + dt*beta1*w1 in stage S2)
@@ -190,20 +289,38 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     if constexpr (G::FORM1) return f + dtb1 * w;
     else return f;
   };
+  auto to_t = [&](double val, double mu) -> T { return MIXED ? (T)(val - mu) : (T)val; };
   auto store_plane = [&](int p) {
     T* dst = ring + slot_of(p) * SLOT;
+    T2* dst2 = reinterpret_cast<T2*>(dst);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double mu = MIXED ? a.means[v] : 0.0;
+    for (int pl = 0; pl < NPL; ++pl) {
+      const int va = LY::v0(pl), vb = LY::v1(pl);
+      const double mua = MIXED ? a.means[va] : 0.0;
+      const double mub = MIXED ? a.means[vb] : 0.0;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const double val = staged(pre[v * K + k], prw[G::FORM1 ? v * K + k : 0]);
-        dst[v * VS + tid + k * NTOT] = MIXED ? (T)(val - mu) : (T)val;
+        const T ea = to_t(staged(pre[va * K + k], prw[G::FORM1 ? va * K + k : 0]), mua);
+        if (pl < NP) {
+          T2 e;
+          e.x = ea;
+          e.y = to_t(staged(pre[vb * K + k], prw[G::FORM1 ? vb * K + k : 0]), mub);
+          dst2[pl * VS + tid + k * NTOT] = e;
+        } else {
+          dst[(2 * NP + pl - NP) * VS + tid + k * NTOT] = ea;
+        }
       }
     }
     if (has_rem) {
-      const double val = staged(prem, prwm);
-      dst[rl] = MIXED ? (T)(val - rmu) : (T)val;
+      const T ea = to_t(staged(prem[0], prwm[0]), rmua);
+      if (rem_pair) {
+        T2 e;
+        e.x = ea;
+        e.y = to_t(staged(prem[1], prwm[1]), rmub);
+        dst2[rl] = e;
+      } else {
+        dst[rl] = ea;
+      }
     }
   };
 
@@ -264,67 +381,147 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   const T C0 = (T)PCMX_D2C0, C1 = (T)PCMX_D2C1, C2 = (T)PCMX_D2C2, C3 = (T)PCMX_D2C3;
   const double dtb = a.d_dt ? a.d_dt[0] * a.beta : a.dtb;
 
-  // stencils of variable v (group-local index i) at plane q; carries updated
-  // with plane q.  own64 receives the FP64 value of the own cell (FP64 form).
-  auto derive = [&](auto emit_c, int v, int i, Derivs<T>& d, const T* L0, const T* L1, const T* L2,
-                    const T* L3) {
+  // Stencils at plane q of the variables of one staged plane (PT = T2: a
+  // pair, components 0 and 1 are variables VA and VB with group-local
+  // indices IA_ and IB_; PT = T: a single plane, VB = -1).  Every tap is one
+  // read of the plane; each component's arithmetic is that of a single
+  // variable.  The taps are taken stage by stage (z, xz, yz, x, y, xy), both
+  // components at once, so that each read is dead after its stage; a tap of
+  // a pair that one component needs keeps the other half alive through
+  // sink, which holds the read at full width.  Carries are updated with
+  // plane q.  Returns the own-cell value(s).
+  auto derive = [&](auto emit_c, auto va_c, auto ia_c, auto vb_c, auto ib_c, Derivs<T>& d, T& sink,
+                    const auto* p0, const auto* p1, const auto* p2, const auto* p3) {
+    using PT = std::remove_cv_t<std::remove_pointer_t<decltype(p0)>>;
+    constexpr bool PAIR = std::is_same_v<PT, T2>;
     constexpr bool EMIT = decltype(emit_c)::value;
-    const T* p0 = L0 + v * VS;
-    const T c0 = p0[0];
-    const T xp1 = p0[1], xm1 = p0[-1], xp2 = p0[2], xm2 = p0[-2], xp3 = p0[3], xm3 = p0[-3];
-    const T yp1 = p0[PX], ym1 = p0[-PX], yp2 = p0[2 * PX], ym2 = p0[-2 * PX];
-    const T yp3 = p0[3 * PX], ym3 = p0[-3 * PX];
+    constexpr int VA = decltype(va_c)::value, VB = decltype(vb_c)::value;
+    static_assert(PAIR == (VB >= 0), "pairs evaluate both components");
+    constexpr bool XY = N::xy(VA) || (VB >= 0 && N::xy(VB));
+    constexpr bool XZ = N::xz(VA) || (VB >= 0 && N::xz(VB));
+    constexpr bool YZ = N::yz(VA) || (VB >= 0 && N::yz(VB));
+    // f(c_c, v_c, i_c) for each component
+    auto each = [&](auto f) {
+      f(IC<0>{}, va_c, ia_c);
+      if constexpr (VB >= 0) f(IC<1>{}, vb_c, ib_c);
+    };
+    // keeps the unused half of pair taps that only component C uses (on the
+    // (uu_z, lnrho) pair only: on the others the extra live registers of the
+    // held halves would spill)
+    auto hold = [&](auto c_c, auto need, auto... e) {
+      constexpr int C = decltype(c_c)::value;
+      if constexpr (PAIR && decltype(need)::value && VA == 2) {
+        (hold_reg(sink, pick<1 - C>(e)), ...);
+      }
+    };
+    const PT c0 = p0[0];
+    T rdx[2] = {}, rdy[2] = {}, rdz[2] = {}, rxx[2] = {}, ryy[2] = {}, rzz[2] = {};
+    T rxy[2] = {}, rxz[2] = {}, ryz[2] = {};
     if constexpr (EMIT) {
-      const T* p1 = L1 + v * VS;
-      const T* p2 = L2 + v * VS;
-      const T* p3 = L3 + v * VS;
-      const T zp1 = p1[0], zp2 = p2[0], zp3 = p3[0];
-      const T zm1 = back[i][0], zm2 = back[i][1], zm3 = back[i][2];
-      // der_main (deriv.f90:124-162)
-      const T dx = fx1 * ((T)45.0 * (xp1 - xm1) - (T)9.0 * (xp2 - xm2) + (xp3 - xm3));
-      const T dy = fy1 * ((T)45.0 * (yp1 - ym1) - (T)9.0 * (yp2 - ym2) + (yp3 - ym3));
-      const T dz = fz1 * ((T)45.0 * (zp1 - zm1) - (T)9.0 * (zp2 - zm2) + (zp3 - zm3));
-      T dxx = 0, dyy = 0, dzz = 0;
-      if (N::d2(v)) {  // der2_main (deriv.f90:497-550)
-        dxx = fx2 * (C0 * c0 + C1 * (xp1 + xm1) + C2 * (xp2 + xm2) + C3 * (xp3 + xm3));
-        dyy = fy2 * (C0 * c0 + C1 * (yp1 + ym1) + C2 * (yp2 + ym2) + C3 * (yp3 + ym3));
-        dzz = fz2 * (C0 * c0 + C1 * (zp1 + zm1) + C2 * (zp2 + zm2) + C3 * (zp3 + zm3));
+      // own column: z derivatives (der_main, der2_main; deriv.f90:124-162, 497-550)
+      const PT zp1 = p1[0], zp2 = p2[0], zp3 = p3[0];
+      each([&](auto c_c, auto v_c, auto i_c) {
+        constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value, I = decltype(i_c)::value;
+        const T zm1 = back[I][0], zm2 = back[I][1], zm3 = back[I][2];
+        rdz[C] = fz1 * ((T)45.0 * (pick<C>(zp1) - zm1) - (T)9.0 * (pick<C>(zp2) - zm2) + (pick<C>(zp3) - zm3));
+        if constexpr (N::d2(V))
+          rzz[C] = fz2 * (C0 * pick<C>(c0) + C1 * (pick<C>(zp1) + zm1) + C2 * (pick<C>(zp2) + zm2) +
+                          C3 * (pick<C>(zp3) + zm3));
+      });
+      if constexpr (XZ) {  // forward half of the xz bidiagonal
+        const PT a1 = p1[1], b1 = p1[-1], a2 = p2[2], b2 = p2[-2], a3 = p3[3], b3 = p3[-3];
+        constexpr bool ONE = !(N::xz(VA) && VB >= 0 && N::xz(VB));
+        hold(IC<N::xz(VA) ? 0 : 1>{}, std::bool_constant<ONE>{}, a1, b1, a2, b2, a3, b3);
+        each([&](auto c_c, auto v_c, auto) {
+          constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+          if constexpr (N::xz(V)) {
+            const T plus = (T)270.0 * (pick<C>(a1) - pick<C>(b1)) - (T)27.0 * (pick<C>(a2) - pick<C>(b2)) +
+                           (T)2.0 * (pick<C>(a3) - pick<C>(b3));
+            rxz[C] = (T)P.facxz * (plus - cxz[N::ixz(V) & 1][0]);
+          }
+        });
       }
-      T mxy = 0, mxzv = 0, myzv = 0;
-      if (N::xy(v)) {  // bidiagonal xy, operand order of deriv.f90:1410-1418
-        const T b1 = p0[1 + PX] - p0[-1 + PX] + p0[-1 - PX] - p0[1 - PX];
-        const T b2 = p0[2 + 2 * PX] - p0[-2 + 2 * PX] + p0[-2 - 2 * PX] - p0[2 - 2 * PX];
-        const T b3 = p0[3 + 3 * PX] - p0[-3 + 3 * PX] + p0[-3 - 3 * PX] - p0[3 - 3 * PX];
-        mxy = (T)P.facxy * ((T)270.0 * b1 - (T)27.0 * b2 + (T)2.0 * b3);
+      if constexpr (YZ) {  // forward half of the yz bidiagonal
+        const PT a1 = p1[PX], b1 = p1[-PX], a2 = p2[2 * PX], b2 = p2[-2 * PX], a3 = p3[3 * PX], b3 = p3[-3 * PX];
+        constexpr bool ONE = !(N::yz(VA) && VB >= 0 && N::yz(VB));
+        hold(IC<N::yz(VA) ? 0 : 1>{}, std::bool_constant<ONE>{}, a1, b1, a2, b2, a3, b3);
+        each([&](auto c_c, auto v_c, auto) {
+          constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+          if constexpr (N::yz(V)) {
+            const T plus = (T)270.0 * (pick<C>(a1) - pick<C>(b1)) - (T)27.0 * (pick<C>(a2) - pick<C>(b2)) +
+                           (T)2.0 * (pick<C>(a3) - pick<C>(b3));
+            ryz[C] = (T)P.facyz * (plus - cyz[N::iyz(V) & 1][0]);
+          }
+        });
       }
-      if (N::xz(v)) {
-        const T plus = (T)270.0 * (p1[1] - p1[-1]) - (T)27.0 * (p2[2] - p2[-2]) + (T)2.0 * (p3[3] - p3[-3]);
-        mxzv = (T)P.facxz * (plus - cxz[N::ixz(v) & 1][0]);
-      }
-      if (N::yz(v)) {
-        const T plus = (T)270.0 * (p1[PX] - p1[-PX]) - (T)27.0 * (p2[2 * PX] - p2[-2 * PX]) +
-                       (T)2.0 * (p3[3 * PX] - p3[-3 * PX]);
-        myzv = (T)P.facyz * (plus - cyz[N::iyz(v) & 1][0]);
-      }
-      file_derivs<T, MAG, ENT>(d, v, dx, dy, dz, dxx, dyy, dzz, mxy, mxzv, myzv);
     }
-    // carry plane q forward (minus parts of the z-mixed derivatives of the
-    // next three planes, and the own column's z window)
-    if (N::xz(v)) {
-      T* m = cxz[N::ixz(v) & 1];
-      m[0] = m[1] + (T)270.0 * (xp1 - xm1);
-      m[1] = m[2] - (T)27.0 * (xp2 - xm2);
-      m[2] = (T)2.0 * (xp3 - xm3);
+    {  // x: derivatives, and the minus half of the xz bidiagonal carried to planes q+1..q+3
+      const PT xp1 = p0[1], xm1 = p0[-1], xp2 = p0[2], xm2 = p0[-2], xp3 = p0[3], xm3 = p0[-3];
+      each([&](auto c_c, auto v_c, auto) {
+        constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+        const T g1 = pick<C>(xp1) - pick<C>(xm1), g2 = pick<C>(xp2) - pick<C>(xm2), g3 = pick<C>(xp3) - pick<C>(xm3);
+        if constexpr (EMIT) {
+          rdx[C] = fx1 * ((T)45.0 * g1 - (T)9.0 * g2 + g3);
+          if constexpr (N::d2(V))
+            rxx[C] = fx2 * (C0 * pick<C>(c0) + C1 * (pick<C>(xp1) + pick<C>(xm1)) +
+                            C2 * (pick<C>(xp2) + pick<C>(xm2)) + C3 * (pick<C>(xp3) + pick<C>(xm3)));
+        }
+        if constexpr (N::xz(V)) {
+          T* m = cxz[N::ixz(V) & 1];
+          m[0] = m[1] + (T)270.0 * g1;
+          m[1] = m[2] - (T)27.0 * g2;
+          m[2] = (T)2.0 * g3;
+        }
+      });
     }
-    if (N::yz(v)) {
-      T* m = cyz[N::iyz(v) & 1];
-      m[0] = m[1] + (T)270.0 * (yp1 - ym1);
-      m[1] = m[2] - (T)27.0 * (yp2 - ym2);
-      m[2] = (T)2.0 * (yp3 - ym3);
+    {  // y: derivatives, and the minus half of the yz bidiagonal
+      const PT yp1 = p0[PX], ym1 = p0[-PX], yp2 = p0[2 * PX], ym2 = p0[-2 * PX], yp3 = p0[3 * PX], ym3 = p0[-3 * PX];
+      each([&](auto c_c, auto v_c, auto) {
+        constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+        const T g1 = pick<C>(yp1) - pick<C>(ym1), g2 = pick<C>(yp2) - pick<C>(ym2), g3 = pick<C>(yp3) - pick<C>(ym3);
+        if constexpr (EMIT) {
+          rdy[C] = fy1 * ((T)45.0 * g1 - (T)9.0 * g2 + g3);
+          if constexpr (N::d2(V))
+            ryy[C] = fy2 * (C0 * pick<C>(c0) + C1 * (pick<C>(yp1) + pick<C>(ym1)) +
+                            C2 * (pick<C>(yp2) + pick<C>(ym2)) + C3 * (pick<C>(yp3) + pick<C>(ym3)));
+        }
+        if constexpr (N::yz(V)) {
+          T* m = cyz[N::iyz(V) & 1];
+          m[0] = m[1] + (T)270.0 * g1;
+          m[1] = m[2] - (T)27.0 * g2;
+          m[2] = (T)2.0 * g3;
+        }
+      });
     }
-    back[i][2] = back[i][1];
-    back[i][1] = back[i][0];
-    back[i][0] = c0;
+    if constexpr (EMIT) {
+      if constexpr (XY) {  // bidiagonal xy, operand order of deriv.f90:1410-1418
+        T bk[2][3] = {};
+#pragma unroll
+        for (int k = 1; k <= 3; ++k) {
+          const PT pp = p0[k + k * PX], mp = p0[-k + k * PX], mm = p0[-k - k * PX], pm = p0[k - k * PX];
+          each([&](auto c_c, auto v_c, auto) {
+            constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+            if constexpr (N::xy(V)) bk[C][k - 1] = pick<C>(pp) - pick<C>(mp) + pick<C>(mm) - pick<C>(pm);
+          });
+        }
+        each([&](auto c_c, auto v_c, auto) {
+          constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+          if constexpr (N::xy(V))
+            rxy[C] = (T)P.facxy * ((T)270.0 * bk[C][0] - (T)27.0 * bk[C][1] + (T)2.0 * bk[C][2]);
+        });
+      }
+      each([&](auto c_c, auto v_c, auto) {
+        constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
+        file_derivs<T, MAG, ENT>(d, V, rdx[C], rdy[C], rdz[C], rxx[C], ryy[C], rzz[C], rxy[C], rxz[C], ryz[C]);
+      });
+    }
+    // the own column's z window moves up one plane
+    each([&](auto c_c, auto, auto i_c) {
+      constexpr int C = decltype(c_c)::value, I = decltype(i_c)::value;
+      back[I][2] = back[I][1];
+      back[I][1] = back[I][0];
+      back[I][0] = pick<C>(c0);
+    });
     return c0;
   };
 
@@ -336,10 +533,22 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   auto iteration = [&](auto emit_c, auto g_c, int q) {
     constexpr bool EMIT = decltype(emit_c)::value;
     constexpr int GRP = decltype(g_c)::value;
-    const T* L0 = ring + slot_of(q) * SLOT + c;
-    const T* L1 = ring + slot_of(q + 1) * SLOT + c;
-    const T* L2 = ring + slot_of(q + 2) * SLOT + c;
-    const T* L3 = ring + slot_of(q + 3) * SLOT + c;
+    const T* L0 = ring + slot_of(q) * SLOT;  // ring slots of planes q .. q+3
+    const T* L1 = ring + slot_of(q + 1) * SLOT;
+    const T* L2 = ring + slot_of(q + 2) * SLOT;
+    const T* L3 = ring + slot_of(q + 3) * SLOT;
+    // own-cell pointers into pair plane j / single plane sp of the four slots
+    auto dpair = [&](auto j_c, auto va_c, auto ia_c, auto vb_c, auto ib_c, Derivs<T>& d, T& sink) {
+      constexpr int j = decltype(j_c)::value;
+      return derive(emit_c, va_c, ia_c, vb_c, ib_c, d, sink, pair_at(L0, j) + c, pair_at(L1, j) + c,
+                    pair_at(L2, j) + c, pair_at(L3, j) + c);
+    };
+    auto dsingle = [&](auto sp_c, auto v_c, auto i_c, Derivs<T>& d, T& sink) {
+      constexpr int sp = decltype(sp_c)::value;
+      return derive(emit_c, v_c, i_c, IC<-1>{}, IC<-1>{}, d, sink, single_at(L0, sp) + c, single_at(L1, sp) + c,
+                    single_at(L2, sp) + c, single_at(L3, sp) + c);
+    };
+    T sink = (T)0;
     // ---------------- phase 1 ----------------
     double wcur[NGV + NOWNW], fcur[NGV + NOWNF];
 #pragma unroll
@@ -358,19 +567,26 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     }
     if constexpr (GRP == 0) {
       Derivs<T> d;
-#pragma unroll
-      for (int i = 0; i < NF; ++i) {
-        const int v = var_of(0, i);
-        const T c0 = derive(emit_c, v, i, d, L0, L1, L2, L3);
-        if constexpr (EMIT) {
-          ff[i] = MIXED ? fcur[i] : (double)c0;
-          const T val = (T)ff[i];
-          if (v < 3) d.u[v] = val;
-          else if (v == 3) d.lnrho = val;
-          else d.ss = val;
-        }
+      T cv[5];
+      if constexpr (PAIRS) {
+        const T2 c01 = dpair(IC<0>{}, IC<0>{}, IC<0>{}, IC<1>{}, IC<1>{}, d, sink);
+        const T2 c23 = dpair(IC<1>{}, IC<2>{}, IC<2>{}, IC<3>{}, IC<3>{}, d, sink);
+        cv[0] = c01.x, cv[1] = c01.y, cv[2] = c23.x, cv[3] = c23.y;
+      } else {
+        cv[0] = dsingle(IC<0>{}, IC<0>{}, IC<0>{}, d, sink);
+        cv[1] = dsingle(IC<1>{}, IC<1>{}, IC<1>{}, d, sink);
+        cv[2] = dsingle(IC<2>{}, IC<2>{}, IC<2>{}, d, sink);
+        cv[3] = dsingle(IC<3>{}, IC<3>{}, IC<3>{}, d, sink);
       }
+      if constexpr (ENT) cv[4] = dsingle(IC<LY::SSS>{}, IC<ISS>{}, IC<4>{}, d, sink);
+      asm volatile("" ::"v"(sink));
       if constexpr (EMIT) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) ff[i] = MIXED ? fcur[i] : (double)cv[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) d.u[j] = (T)ff[j];
+        d.lnrho = (T)ff[3];
+        if constexpr (ENT) d.ss = (T)ff[4];
 #pragma unroll
         for (int i = 0; i < NF; ++i) wf[i] = G::READW ? a.alpha * wcur[i] : 0.0;
         const T rho1 = pexp<T>(-d.lnrho);
@@ -379,25 +595,39 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     } else if constexpr (MAG) {
       Derivs<T> d;
       double fa[3];
+      T cv[3];
+      if constexpr (PAIRS) {
+        const T2 cxy = dpair(IC<2>{}, IC<IA>{}, IC<0>{}, IC<IA + 1>{}, IC<1>{}, d, sink);
+        cv[0] = cxy.x, cv[1] = cxy.y;
+      } else {
+        cv[0] = dsingle(IC<IA>{}, IC<IA>{}, IC<0>{}, d, sink);
+        cv[1] = dsingle(IC<IA + 1>{}, IC<IA + 1>{}, IC<1>{}, d, sink);
+      }
+      cv[2] = dsingle(IC<LY::SAZ>{}, IC<IA + 2>{}, IC<2>{}, d, sink);
+      asm volatile("" ::"v"(sink));
+      if constexpr (EMIT) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const int v = IA + i;
-        const T c0 = derive(emit_c, v, i, d, L0, L1, L2, L3);
-        if constexpr (EMIT) {
-          fa[i] = MIXED ? fcur[i] : (double)c0;
+        for (int i = 0; i < 3; ++i) {
+          fa[i] = MIXED ? fcur[i] : (double)cv[i];
           d.a[i] = (T)fa[i];
         }
-      }
-      if constexpr (EMIT) {
         T lnrho;
         if constexpr (MIXED) {
 #pragma unroll
           for (int j = 0; j < 3; ++j) d.u[j] = (T)fcur[NGV + j];
           lnrho = (T)fcur[NGV + 3];
         } else {
+          if constexpr (PAIRS) {
+            const T2 u01 = pair_at(L0, 0)[c], u2l = pair_at(L0, 1)[c];
+            d.u[0] = u01.x;
+            d.u[1] = u01.y;
+            d.u[2] = u2l.x;
+            lnrho = u2l.y;
+          } else {
 #pragma unroll
-          for (int j = 0; j < 3; ++j) d.u[j] = L0[j * VS];
-          lnrho = L0[3 * VS];
+            for (int j = 0; j < 3; ++j) d.u[j] = single_at(L0, j)[c];
+            lnrho = single_at(L0, 3)[c];
+          }
         }
         const T rho1 = pexp<T>(-lnrho);
         double wa[3];
@@ -491,7 +721,8 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 // L2 then serves the rims).
 template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
-  __shared__ T ring[zs_ring_elems<T, MAG, ENT, TL>()];
+  constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, false>();
+  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS>()];
   __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
   const long b = blockIdx.x;
   const long wid = zp.xcd_remap ? (b % 8) * (zp.nblocks / 8) + b / 8 : b;
@@ -502,7 +733,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   double red[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-  zsplit_column<T, MAG, ENT, POLY, ST, TL>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring, xch, red);
+  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring, xch,
+                                                 red);
   if constexpr (StageT<ST>::DIAG) {
     if (a.partials) block_reduce_store<zs_threads<MAG>()>(red, a.partials, (long)blockIdx.x);
   }
@@ -527,7 +759,8 @@ struct EnsWork {
 template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsBox* boxes, const EnsWork* work,
                                                                       int nwork, double* partials) {
-  __shared__ T ring[zs_ring_elems<T, MAG, ENT, TL>()];
+  constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, true>();
+  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS>()];
   __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
   const int nb = gridDim.x;
   // with a block count divisible by 8 every XCD holds nb/8 blocks: XCD x then
@@ -544,7 +777,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsB
     double red[NRED];
 #pragma unroll
     for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-    zsplit_column<T, MAG, ENT, POLY, ST, TL>(a, P, it.x0, it.y0, it.z0, it.lz, ring, xch, red);
+    zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS>(a, P, it.x0, it.y0, it.z0, it.lz, ring, xch, red);
     if constexpr (StageT<ST>::DIAG) {
       if (partials) block_reduce_store<zs_threads<MAG>()>(red, partials, (long)w);
     }
