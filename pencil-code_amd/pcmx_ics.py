@@ -11,6 +11,8 @@
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 
@@ -42,6 +44,72 @@ def bandlimited(ncomp, n, kmin, kmax, rms, seed):
         spec[idx] = 0
     cur = np.sqrt(np.mean(np.sum(out * out, axis=0)))
     return out * (rms / cur) if cur > 0 else out
+
+
+def _shell_modes(nx, ny, nz, kmin, kmax):
+    """Half-spectrum wavenumbers (kz, ky, kx) with kmin <= |k| <= kmax, in
+    the order np.nonzero visits them on the (nz, ny, nx//2+1) grid of FFT
+    wavenumbers (so that random draws land on the same modes as in
+    bandlimited)."""
+    def axis(n):  # wavenumbers with |k| <= kmax in ascending FFT index order
+        return list(range(0, kmax + 1)) + list(range(-kmax, 0))
+    modes = []
+    for kz in axis(nz):
+        for ky in axis(ny):
+            for kx in range(0, kmax + 1):
+                kk = math.sqrt(float(kx * kx) + float(ky * ky) + float(kz * kz))
+                if kmin <= kk <= kmax:
+                    modes.append((kz, ky, kx))
+    return modes
+
+
+def bandlimited_into(out, kmin, kmax, rms, seed, zslab=64, progress=None):
+    """The field of bandlimited(out.shape[0], (nx, ny, nz), kmin, kmax, rms,
+    seed), written into out (ncomp, nz, ny, nx; a view is fine) without a
+    full-size FFT: with only |k| <= kmax present the inverse transform is a
+    sum over few modes, done separably (x, then y, then z slabs as two real
+    matrix products), so memory stays at a few slabs.  Needs every extent
+    > 2*kmax."""
+    ncomp, nz, ny, nx = out.shape
+    if min(nx, ny, nz) <= 2 * kmax:
+        raise ValueError("grid too small for the shell")
+    rng = np.random.default_rng(seed)
+    modes = _shell_modes(nx, ny, nz, kmin, kmax)
+    m = len(modes)
+    ks = list(range(-kmax, kmax + 1))        # z and y wavenumbers, array index k + kmax
+    two_pi = 2.0 * math.pi
+    fx = np.exp(1j * two_pi * np.outer(np.arange(kmax + 1), np.arange(nx)) / nx)     # (kx, x)
+    fy = np.exp(1j * two_pi * np.outer(ks, np.arange(ny)) / ny)                       # (ky, y)
+    fz = np.exp(1j * two_pi * np.outer(ks, np.arange(nz)) / nz)                       # (kz, z)
+    for c in range(ncomp):
+        coef = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+        cm = np.zeros((len(ks), len(ks), kmax + 1), dtype=np.complex128)
+        for (kz, ky, kx), v in zip(modes, coef):
+            # irfft counts kx > 0 twice (conjugate half) and the real part of kx = 0 once
+            cm[kz + kmax, ky + kmax, kx] = v * (2.0 if kx > 0 else 1.0)
+        g = cm @ fx                                          # (kz, ky, x)
+        h = np.einsum("jy,ijx->iyx", fy, g).reshape(len(ks), ny * nx)
+        # Re(sum_kz e^{i kz z} h_kz) = [cos | -sin](z, kz) @ [Re h ; Im h]
+        hs = np.concatenate([h.real, h.imag])                # (2 kz, y x)
+        del g, h
+        buf = np.empty((zslab, ny * nx))
+        for z0 in range(0, nz, zslab):
+            z1 = min(nz, z0 + zslab)
+            a = np.concatenate([fz[:, z0:z1].real, -fz[:, z0:z1].imag]).T
+            np.matmul(a, hs, out=buf[: z1 - z0])
+            out[c, z0:z1] = buf[: z1 - z0].reshape(z1 - z0, ny, nx)
+        if progress:
+            progress(f"band-limited component {c + 1}/{ncomp} done")
+    ssq = 0.0
+    for c in range(ncomp):
+        for z0 in range(0, nz, zslab):
+            blk = out[c, z0:min(nz, z0 + zslab)]
+            ssq += float(np.einsum("zyx,zyx->", blk, blk))
+    cur = math.sqrt(ssq / (float(nx) * ny * nz))
+    if cur > 0:
+        for c in range(ncomp):
+            for z0 in range(0, nz, zslab):
+                out[c, z0:min(nz, z0 + zslab)] *= rms / cur
 
 
 def config_state(kind, n, seed, lmagnetic=False, lentropy=False):
