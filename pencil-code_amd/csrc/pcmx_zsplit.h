@@ -170,19 +170,48 @@ __device__ __forceinline__ void hold_reg(T& sink, T v) {
   asm("" : "+v"(sink) : "v"(v));
 }
 
-// Elements (of the state type) of the LDS ring.
-template <typename T, bool MAG, bool ENT, class TL, bool PAIRS>
-constexpr int zs_ring_elems() {
-  using LY = ZLayout<ENT, MAG, PAIRS>;
-  return ZM_RING * LY::WIDTH * ZStage<TL, LY::NP, LY::NS, zs_threads<MAG>()>::VS;
+// Values per cell the field half passes to the flow half: J x B / rho, then
+// J^2 (entropy) and the Alfven Courant term (stages that reduce dt).
+template <bool MAG, bool ENT, int ST>
+constexpr int zs_nxc() {
+  return MAG ? 3 + (ENT ? 1 : 0) + (StageT<ST>::DT ? 1 : 0) : 0;
 }
 
-constexpr int ZS_NXC = 5;  // exchange slots per cell: jxbr[3], j2, advec_va2
+// Ring slots.  Four hold planes q..q+3; the refill with plane q+4 must then
+// wait until every wave has read plane q, a second barrier per plane.  With
+// a fifth slot, plane q+4 goes into the slot of plane q-1 before the one
+// barrier, and a double-buffered exchange lets the flow half finish plane q
+// while the field half starts plane q+1.  Five slots where they fit in LDS
+// (with the magnetic field: one block per CU in any case).
+constexpr int ZS_LDS_BYTES = 163840;
+template <typename T, bool MAG, bool ENT, int ST, class TL, bool PAIRS>
+constexpr int zs_ring_slots() {
+  if constexpr (!MAG) {
+    return 4;
+  } else {
+    using LY = ZLayout<ENT, MAG, PAIRS>;
+    constexpr long slot = (long)LY::WIDTH * ZStage<TL, LY::NP, LY::NS, zs_threads<MAG>()>::VS * sizeof(T);
+    constexpr long xch2 = 2L * zs_nxc<MAG, ENT, ST>() * TL::NT * sizeof(T);
+    constexpr long red = (long)(zs_threads<MAG>() / 64) * NRED * sizeof(double);
+    return 5 * slot + xch2 + red <= ZS_LDS_BYTES ? 5 : 4;
+  }
+}
+
+// Elements (of the state type) of the LDS ring and of the exchange buffer.
+template <typename T, bool MAG, bool ENT, class TL, bool PAIRS, int RING>
+constexpr int zs_ring_elems() {
+  using LY = ZLayout<ENT, MAG, PAIRS>;
+  return RING * LY::WIDTH * ZStage<TL, LY::NP, LY::NS, zs_threads<MAG>()>::VS;
+}
+template <bool MAG, bool ENT, int ST, class TL, int RING>
+constexpr int zs_xch_elems() {
+  return MAG ? (RING == 5 ? 2 : 1) * zs_nxc<MAG, ENT, ST>() * TL::NT : 1;
+}
 
 // Marches one tile column (x0, y0, z0 .. z0+lz-1) of one box through the
 // RK stage ST.  ring and xch are the block's LDS buffers; the column's
 // reduction terms are added to red (all threads of the block take part).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS>
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS, int RING>
 __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int x0, int y0, int z0, int lz,
                                               T* ring, T* xch, double (&red)[NRED]) {
   using S = Slots<ENT, MAG>;
@@ -219,7 +248,13 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     zz += (zz < 0) ? nz : 0;
     return (long)zz * nxy;
   };
-  auto slot_of = [&](int p) -> int { return (p - z0 + 3) & 3; };
+  static_assert(RING == 4 || RING == 5, "four or five ring slots");
+  auto slot_of = [&](int p) -> int { return RING == 4 ? (p - z0 + 3) & 3 : (p - z0 + 3) % 5; };
+  // exchange buffer: values jxbr[0..2], [j2], [advec_va2]; two buffers
+  // (by plane parity) with five ring slots
+  constexpr int NXC = zs_nxc<MAG, ENT, ST>();
+  constexpr int XJ2 = 3, XVA = 3 + (ENT ? 1 : 0);
+  auto xbuf = [&](int q) -> T* { return xch + (RING == 5 ? (q & 1) * NXC * TL::NT : 0); };
   // offset inside a z plane of element ee of the tile's haloed plane
   auto plane_off = [&](int ee) -> unsigned {
     const int row = ee / PX, col = ee % PX;
@@ -359,8 +394,8 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     }
   };
 
-  // ---- prime the ring with planes z0-3 .. z0, start plane z0+1 ----
-  for (int p = z0 - 3; p <= z0; ++p) {
+  // ---- prime the ring with planes z0-3 .. z0+RING-4, start the next ----
+  for (int p = z0 - 3; p <= z0 + RING - 4; ++p) {
     load_plane(p);
     store_plane(p);
   }
@@ -530,6 +565,20 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   FlowKeep<T> keep;
   double u2c = 0.0;
 
+  // refill, after the flow half's second phase: the next plane replaces
+  // plane q, which every wave has read before the barrier of this
+  // iteration.  Four slots: plane q+4, read from the next iteration on, so a
+  // second barrier follows.  Five slots: plane q+5, first read two
+  // iterations on, past the next barrier, so none is needed.  Then the
+  // plane after it starts loading (past the last plane the block needs, the
+  // last one is fetched again, an L2 hit, so that every iteration issues the
+  // same memory operations).
+  constexpr int AHEAD = RING;  // plane stored by iteration q: q + AHEAD
+  auto refill = [&](int q) {
+    store_plane(q + AHEAD);
+    load_plane(q + AHEAD + 1 <= zlast + 3 ? q + AHEAD + 1 : zlast + 3);
+  };
+
   auto iteration = [&](auto emit_c, auto g_c, int q) {
     constexpr bool EMIT = decltype(emit_c)::value;
     constexpr int GRP = decltype(g_c)::value;
@@ -636,10 +685,11 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
         FieldOut<T> fo;
         double b2 = 0.0;
         field_point<T, ENT, DT>(d, rho1, P, wa, fo, &b2);
+        T* xo = xbuf(q);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) xch[j * TL::NT + t] = fo.jxbr[j];
-        if constexpr (ENT) xch[3 * TL::NT + t] = fo.j2;
-        if constexpr (DT) xch[4 * TL::NT + t] = fo.advec_va2;
+        for (int j = 0; j < 3; ++j) xo[j * TL::NT + t] = fo.jxbr[j];
+        if constexpr (ENT) xo[XJ2 * TL::NT + t] = fo.j2;
+        if constexpr (DT) xo[XVA * TL::NT + t] = fo.advec_va2;
         if constexpr (G::UPDATE) {
 #pragma unroll
           for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(fa[i] + dtb * wa[i], &vplane(a.fout, IA + i, q)[own]);
@@ -662,10 +712,11 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
       if constexpr (GRP == 0) {
         FieldOut<T> fo{};
         if constexpr (MAG) {
+          const T* xi = xbuf(q);
 #pragma unroll
-          for (int j = 0; j < 3; ++j) fo.jxbr[j] = xch[j * TL::NT + t];
-          if constexpr (ENT) fo.j2 = xch[3 * TL::NT + t];
-          if constexpr (DT) fo.advec_va2 = xch[4 * TL::NT + t];
+          for (int j = 0; j < 3; ++j) fo.jxbr[j] = xi[j * TL::NT + t];
+          if constexpr (ENT) fo.j2 = xi[XJ2 * TL::NT + t];
+          if constexpr (DT) fo.advec_va2 = xi[XVA * TL::NT + t];
         }
         double dt1 = 0.0;
         flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
@@ -686,12 +737,8 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
         }
       }
     }
-    // refill: plane q+4 replaces plane q in the ring; start plane q+5 (past
-    // the last plane the block needs, the last one is fetched again, an L2
-    // hit, so that every iteration issues the same memory operations)
-    store_plane(q + 4);
-    load_plane(q + 5 <= zlast + 3 ? q + 5 : zlast + 3);
-    __syncthreads();
+    refill(q);
+    if constexpr (RING == 4) __syncthreads();
   };
 
   // Each wave group runs its own copy of the marching loop: inside a loop
@@ -702,7 +749,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   auto run = [&](auto g_c) {
     // same issue order as the loop body: own-cell loads, then the plane
     prefetch_own(g_c, z0);
-    load_plane(z0 + 1);
+    load_plane(z0 + RING - 3);
     __syncthreads();
     for (int q = z0 - 3; q < z0; ++q) iteration(std::false_type{}, g_c, q);
     for (int q = z0; q <= zlast; ++q) iteration(std::true_type{}, g_c, q);
@@ -722,8 +769,9 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
   constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, false>();
-  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS>()];
-  __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
+  constexpr int RING = zs_ring_slots<T, MAG, ENT, ST, TL, PAIRS>();
+  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS, RING>()];
+  __shared__ T xch[zs_xch_elems<MAG, ENT, ST, TL, RING>()];
   const long b = blockIdx.x;
   const long wid = zp.xcd_remap ? (b % 8) * (zp.nblocks / 8) + b / 8 : b;
   const int txt = (int)(wid % zp.ntx);
@@ -733,8 +781,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   double red[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring, xch,
-                                                 red);
+  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring,
+                                                       xch, red);
   if constexpr (StageT<ST>::DIAG) {
     if (a.partials) block_reduce_store<zs_threads<MAG>()>(red, a.partials, (long)blockIdx.x);
   }
@@ -760,8 +808,9 @@ template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsBox* boxes, const EnsWork* work,
                                                                       int nwork, double* partials) {
   constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, true>();
-  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS>()];
-  __shared__ T xch[MAG ? ZS_NXC * TL::NT : 1];
+  constexpr int RING = zs_ring_slots<T, MAG, ENT, ST, TL, PAIRS>();
+  __shared__ __attribute__((aligned(16))) T ring[zs_ring_elems<T, MAG, ENT, TL, PAIRS, RING>()];
+  __shared__ T xch[zs_xch_elems<MAG, ENT, ST, TL, RING>()];
   const int nb = gridDim.x;
   // with a block count divisible by 8 every XCD holds nb/8 blocks: XCD x then
   // takes the x-th eighth of the list; otherwise plain striding
@@ -777,7 +826,7 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsB
     double red[NRED];
 #pragma unroll
     for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-    zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS>(a, P, it.x0, it.y0, it.z0, it.lz, ring, xch, red);
+    zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING>(a, P, it.x0, it.y0, it.z0, it.lz, ring, xch, red);
     if constexpr (StageT<ST>::DIAG) {
       if (partials) block_reduce_store<zs_threads<MAG>()>(red, partials, (long)w);
     }
