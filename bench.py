@@ -94,7 +94,8 @@ def run_form(f, par, steps, warmup, dt, keep_state=False, tune=True):
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     return dict(value=ncell * steps / sec / 1e6, sec=sec, ms_per_step=sec * 1e3 / steps, kernel_ms=kms,
                 nkernels=nk, achieved=achieved, urms=float(red[0]), brms=float(red[2]), t=t, dt=dt,
-                state=state, plan=(f"{plan[1]}x{256 // plan[1]} tiles, {plan[2]}-plane z chunks" if plan else None))
+                state=state, plan=(f"{plan[1]}x{256 // plan[1]} tiles, {plan[2]}-plane z chunks" if plan
+                                   else (None if tune else "default plan (not tuned)")))
 
 
 def measured_traffic(n):
@@ -324,6 +325,9 @@ def main():
     ap.add_argument("--kernel", type=int, default=None)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default plan (32x8 tiles, chunk length from the grid) instead of tuning; "
+                         "counter profiles use it so that every pass runs the same plan")
     ap.add_argument("--config", type=int, default=0, choices=[0, 1, 2, 3, 4, 5],
                     help="run reference configuration K instead of the default bench line")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -352,13 +356,14 @@ def main():
 
     kw = dict(lmagnetic=True, nu=1e-3, eta=1e-3, kernel=args.kernel)
     log("bench: FP64 form")
-    r64 = run_form(f, pcmx.make_params(n, precision=pcmx.PREC_FP64, **kw), args.steps, args.warmup, args.dt)
+    r64 = run_form(f, pcmx.make_params(n, precision=pcmx.PREC_FP64, **kw), args.steps, args.warmup, args.dt,
+                   tune=not args.no_tune)
     log(f"bench: FP64 { {k: v for k, v in r64.items() if k != 'state'} }")
     rmx = None
     if not args.no_mixed:
         log("bench: mixed form")
         rmx = run_form(f, pcmx.make_params(n, precision=pcmx.PREC_MIXED, **kw), args.steps, args.warmup,
-                       args.dt)
+                       args.dt, tune=not args.no_tune)
         log(f"bench: mixed { {k: v for k, v in rmx.items() if k != 'state'} }")
     del f
     if args.copy_calib:
