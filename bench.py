@@ -237,6 +237,55 @@ def run_config4_large(n):
     return out
 
 
+def run_io(n):
+    """Snapshot output at n^3 (isothermal MHD, 7 variables): var.dat written
+    from the device (pcmx_write_snapshot, one variable at a time through
+    pinned chunks) against the route of the reference's GPU module, the
+    whole f-array copied to the host (copy_farray_from_GPU) and written by
+    the host (the same record layout, pcmx_write_record).  Both files are
+    compared byte for byte and deleted."""
+    import tempfile
+
+    par = pcmx.make_params(n, lmagnetic=True, nu=1e-3, eta=1e-3)
+    f = pcmx.farray_from_interior(pcmx_ics.config_state("shells", n, 7, lmagnetic=True))
+    d = tempfile.mkdtemp(prefix="pcmx_io_", dir=os.environ.get("TMPDIR", "/tmp"))
+    a, b = os.path.join(d, "var_device.dat"), os.path.join(d, "var_host.dat")
+    pcmx.initialize_gpu(f, par, nt=1)
+    try:
+        t, _ = pcmx.time_step(1, 0.0, 1e-3)
+        t0 = time.perf_counter()
+        pcmx.write_snapshot(a, t)
+        sec_dev = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        host = np.empty_like(f)
+        pcmx.copy_farray_from_gpu(host)
+        sec_copy = time.perf_counter() - t0
+        g = pcmx.NGHOST
+        m = [np.arange(1, k + 2 * g + 1) - 3.5 for k in (n, n, n)]
+        grid = [np.array([t])] + [-math.pi + mm * par.dx for mm in m] + [np.array([par.dx, par.dy, par.dz])]
+        pcmx.write_record(b, [host[:par.mvar]])
+        pcmx.write_record(b, grid, append=True)
+        sec_host = time.perf_counter() - t0
+    finally:
+        pcmx.finalize_gpu()
+    size = os.path.getsize(a)
+    same = size == os.path.getsize(b)
+    if same:
+        with open(a, "rb") as fa, open(b, "rb") as fb:
+            while same:
+                x, y = fa.read(1 << 26), fb.read(1 << 26)
+                same = x == y
+                if not x:
+                    break
+    for p in (a, b):
+        os.remove(p)
+    os.rmdir(d)
+    return {"io": {"grid": n, "bytes": size, "identical": same,
+                   "device_snapshot_s": round(sec_dev, 3), "device_snapshot_GBps": round(size / sec_dev / 1e9, 2),
+                   "copy_then_write_s": round(sec_host, 3), "of_which_copy_s": round(sec_copy, 3),
+                   "host_f_array_bytes": int(f.nbytes)}}
+
+
 ENSEMBLE_DESC = ("ensemble of 64 isothermal MHD boxes, sizes uniform in {32,48,64,96}^3 (side 2pi), "
                  "eta log-uniform in [1e-3,1e-2], Pm=nu/eta log-uniform in [0.5,10], band-limited uu/aa with "
                  "distinct seeds, fixed dt per box, 200 steps; mixed form = FP64 state with FP32 RHS")
@@ -325,6 +374,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=None)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--io", type=int, default=0, metavar="N",
+                    help="time var.dat output at N^3: from the device against copy-back plus host write")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default plan (32x8 tiles, chunk length from the grid) instead of tuning; "
                          "counter profiles use it so that every pass runs the same plan")
@@ -338,6 +389,9 @@ def main():
               "decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
     threads = min(16, os.cpu_count() or 1)
+    if args.io:
+        print(json.dumps(run_io(args.io)), flush=True)
+        return
     if args.config:
         if args.config == 5:
             res = run_config5(threads)

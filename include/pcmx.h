@@ -151,6 +151,38 @@ int pcmx_kernel_timing(double* out);
 int pcmx_set_timing(int enable);
 
 /* ------------------------------------------------------------------------
+ * Snapshots written from the device.  No counterpart in the reference's
+ * module GPU, which copies the whole f-array back (copy_farray_from_GPU,
+ * equ.f90:307-317) before the host writes it (output_snap,
+ * src/io_dist.f90:108-199).
+ *
+ * pcmx_write_snapshot writes the evolved variables in the reference's
+ * var.dat layout for one process, without the persistent-data trailer
+ * (lpersist=F):
+ *   record 1: f(mx,my,mz,1:mvar) real(8) with periodic ghost zones
+ *             (io_dist.f90:187);
+ *   record 2: t, x(1:mx), y(1:my), z(1:mz), dx, dy, dz, real(8)
+ *             (io_dist.f90:195);
+ * x(i) = xyz0[0] + (i - 3.5)*dx for 1-based i: the periodic equidistant grid
+ * of grid.f90:110-216 (the reference computes the same values through its
+ * grid function, so they may differ in the last bits).  The state goes to
+ * the file one variable at a time through two pinned 64 MiB chunks; the
+ * host f-array is not touched.  Records follow the gfortran runtime's
+ * unformatted sequential convention (4-byte byte counts around each record;
+ * records over 2147483639 bytes split into subrecords whose leading count is
+ * negated when another follows and whose trailing count is negated when one
+ * precedes), which the reference's input_snap reads back. */
+int pcmx_write_snapshot(const char* path, double t, const double* xyz0);
+
+/* One record of that convention holding nparts arrays of doubles back to
+ * back, written to path (appended when append != 0).  max_subrecord: the
+ * subrecord length in bytes, 0 for the runtime's 2147483639.  A host-side
+ * helper (time series, grid files, tests of the record layout); needs no
+ * GPU and no pcmx_initialize. */
+int pcmx_write_record(const char* path, int append, const double* const* parts, const long* counts, int nparts,
+                      long max_subrecord);
+
+/* ------------------------------------------------------------------------
  * Ensembles of independent boxes.  No counterpart in the reference's module
  * GPU (one box per run there); for callers that advance many small periodic
  * boxes, e.g. parameter scans.  All boxes share the physics of the params

@@ -10,11 +10,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <cstdint>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
@@ -133,6 +136,9 @@ struct Engine {
   // kernel 3 defers the first substep's update to the second substep: after
   // substep 1, f[cur] still holds f0 and f1 = f0 + dt*beta1*w1 is implicit
   bool pending_f1 = false;
+  // pinned chunks of the device -> file snapshot path
+  double* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
 Engine* g = nullptr;
@@ -296,11 +302,68 @@ void free_engine(Engine* e) {
   if (!e) return;
   for (double* p : {e->f[0], e->f[1], e->w, e->w2, e->partials, e->results, e->staging})
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    if (e->pin[k]) (void)hipHostFree(e->pin[k]);
+    if (e->pin_ev[k]) (void)hipEventDestroy(e->pin_ev[k]);
+  }
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 
 long mw_of(const pcmx_params& p) { return (long)(p.nx + 6) * (p.ny + 6) * (p.nz + 6); }
+
+// One Fortran unformatted sequential record, written as the gfortran runtime
+// writes it: a 4-byte byte count before and after the data; a record longer
+// than max_sub bytes (2147483639 by default) is split into subrecords, each
+// framed the same way, whose leading count is negated when another
+// subrecord follows and whose trailing count is negated when one precedes.
+// The total length is known up front; data may arrive in pieces of any size.
+class FRecord {
+ public:
+  static constexpr uint64_t MAX_SUB = 2147483639ULL;
+  FRecord(FILE* fp, uint64_t total, uint64_t max_sub) : fp_(fp), left_(total), max_sub_(max_sub ? max_sub : MAX_SUB) {
+    begin();
+  }
+  bool put(const void* data, uint64_t n) {
+    const char* c = static_cast<const char*>(data);
+    while (n > 0 && ok_) {
+      if (in_sub_ == sub_len_) {  // only when a piece ends exactly at a subrecord boundary
+        end();
+        begin();
+      }
+      const uint64_t k = std::min(n, sub_len_ - in_sub_);
+      ok_ = ok_ && std::fwrite(c, 1, k, fp_) == k;
+      in_sub_ += k;
+      left_ -= k;
+      c += k;
+      n -= k;
+    }
+    return ok_;
+  }
+  bool finish() {  // closes the last subrecord; every byte must have arrived
+    if (left_ != 0) return false;
+    end();
+    return ok_;
+  }
+
+ private:
+  void marker(int64_t v) {
+    const int32_t m = (int32_t)v;
+    ok_ = ok_ && std::fwrite(&m, 4, 1, fp_) == 1;
+  }
+  void begin() {
+    sub_len_ = std::min(left_, max_sub_);
+    in_sub_ = 0;
+    marker(left_ > sub_len_ ? -(int64_t)sub_len_ : (int64_t)sub_len_);
+  }
+  void end() {
+    marker(continued_ ? -(int64_t)sub_len_ : (int64_t)sub_len_);
+    continued_ = true;
+  }
+  FILE* fp_;
+  uint64_t left_, max_sub_, sub_len_ = 0, in_sub_ = 0;
+  bool continued_ = false, ok_ = true;
+};
 
 }  // namespace
 
@@ -726,6 +789,96 @@ int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag) {
     for (auto& ev : evs) (void)hipEventDestroy(ev);
   }
   return PCMX_OK;
+}
+
+// ------------------------------------------------------------------ snapshots
+
+int pcmx_write_record(const char* path, int append, const double* const* parts, const long* counts, int nparts,
+                      long max_subrecord) {
+  if (!path || nparts < 0 || (nparts > 0 && (!parts || !counts)) || max_subrecord < 0 ||
+      max_subrecord > (long)FRecord::MAX_SUB)
+    return fail(PCMX_ERR_ARG, "pcmx_write_record: bad arguments");
+  uint64_t total = 0;
+  for (int i = 0; i < nparts; ++i) {
+    if (counts[i] < 0 || (counts[i] > 0 && !parts[i])) return fail(PCMX_ERR_ARG, "pcmx_write_record: part %d", i);
+    total += (uint64_t)counts[i] * sizeof(double);
+  }
+  FILE* fp = std::fopen(path, append ? "ab" : "wb");
+  if (!fp) return fail(PCMX_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
+  FRecord rec(fp, total, (uint64_t)max_subrecord);
+  bool ok = true;
+  for (int i = 0; i < nparts && ok; ++i) ok = rec.put(parts[i], (uint64_t)counts[i] * sizeof(double));
+  ok = ok && rec.finish();
+  ok = (std::fclose(fp) == 0) && ok;
+  return ok ? PCMX_OK : fail(PCMX_ERR_ARG, "write to %s failed", path);
+}
+
+int pcmx_write_snapshot(const char* path, double t, const double* xyz0) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (!path || !xyz0) return fail(PCMX_ERR_ARG, "pcmx_write_snapshot: path or xyz0 is NULL");
+  const pcmx_params& p = e.par;
+  const long mw = mw_of(p);
+  constexpr long CHUNK = 8L << 20;  // doubles per pinned chunk (64 MiB)
+  if (!e.staging) HIPCHECK(hipMalloc((void**)&e.staging, sizeof(double) * mw));
+  for (int k = 0; k < 2; ++k) {
+    if (!e.pin[k]) HIPCHECK(hipHostMalloc((void**)&e.pin[k], sizeof(double) * CHUNK, hipHostMallocDefault));
+    if (!e.pin_ev[k]) HIPCHECK(hipEventCreateWithFlags(&e.pin_ev[k], hipEventDisableTiming));
+  }
+  std::unique_ptr<FILE, int (*)(FILE*)> fh(std::fopen(path, "wb"), &std::fclose);  // closed on every return
+  if (!fh) return fail(PCMX_ERR_ARG, "cannot open %s: %s", path, std::strerror(errno));
+  FILE* fp = fh.get();
+  // record 1: f(mx,my,mz,1:mvar) with periodic ghosts (io_dist.f90:187)
+  FRecord rec(fp, (uint64_t)mw * e.nvar * sizeof(double), 0);
+  const double* state = current_state(e);
+  const long nch = (mw + CHUNK - 1) / CHUNK;  // chunks per variable
+  const long nall = nch * e.nvar;
+  auto issue = [&](long c) -> int {  // pack (first chunk of a variable) and copy chunk c
+    const int v = (int)(c / nch);
+    const long off = (c % nch) * CHUNK, n = std::min(CHUNK, mw - off);
+    if (c % nch == 0) {
+      hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, state + (long)v * e.nxyz, e.staging,
+                         p.nx, p.ny, p.nz);
+      HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipMemcpyAsync(e.pin[c & 1], e.staging + off, sizeof(double) * n, hipMemcpyDeviceToHost, e.stream));
+    HIPCHECK(hipEventRecord(e.pin_ev[c & 1], e.stream));
+    return PCMX_OK;
+  };
+  bool ok = true;
+  if (nall > 0 && (rc = issue(0))) return rc;
+  for (long c = 0; c < nall && ok; ++c) {
+    // the copy of chunk c+1 runs while chunk c is written to the file; its
+    // buffer held chunk c-1, already written
+    if (c + 1 < nall && (rc = issue(c + 1))) return rc;
+    HIPCHECK(hipEventSynchronize(e.pin_ev[c & 1]));
+    const long n = std::min(CHUNK, mw - (c % nch) * CHUNK);
+    ok = rec.put(e.pin[c & 1], (uint64_t)n * sizeof(double));
+  }
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  ok = ok && rec.finish();
+  // record 2: t, x(mx), y(my), z(mz), dx, dy, dz (io_dist.f90:195).  The
+  // periodic equidistant grid of grid.f90:110-216: x(i) = x0 + Lx*xi/nx with
+  // xi = i - nghost - 1 + 1/2 (1-based i), i.e. x0 + (i - 3.5)*dx.
+  const int mx = p.nx + 6, my = p.ny + 6, mz = p.nz + 6;
+  std::vector<double> grid;
+  grid.reserve(4 + mx + my + mz);
+  grid.push_back(t);
+  auto coord = [](double x0, int i, double d) {
+#pragma clang fp contract(off)
+    return x0 + (i - 3.5) * d;  // a rounded product, then the sum (no fused multiply-add)
+  };
+  for (int i = 1; i <= mx; ++i) grid.push_back(coord(xyz0[0], i, p.dx));
+  for (int i = 1; i <= my; ++i) grid.push_back(coord(xyz0[1], i, p.dy));
+  for (int i = 1; i <= mz; ++i) grid.push_back(coord(xyz0[2], i, p.dz));
+  grid.push_back(p.dx);
+  grid.push_back(p.dy);
+  grid.push_back(p.dz);
+  FRecord rec2(fp, grid.size() * sizeof(double), 0);
+  ok = ok && rec2.put(grid.data(), grid.size() * sizeof(double)) && rec2.finish();
+  ok = (std::fclose(fh.release()) == 0) && ok;
+  return ok ? PCMX_OK : fail(PCMX_ERR_ARG, "write to %s failed", path);
 }
 
 }  // extern "C"

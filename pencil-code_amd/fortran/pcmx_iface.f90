@@ -99,6 +99,14 @@ module pcmx_iface
       integer(c_int), value :: nsteps, ldiag
       real(c_double), intent(inout) :: t, dt
     end function
+    ! var.dat written from the device state (io_dist.f90:108-199 layout,
+    ! lpersist=F); path NUL-terminated (trim(file)//c_null_char)
+    integer(c_int) function pcmx_write_snapshot(path, t, xyz0) bind(C, name='pcmx_write_snapshot')
+      import :: c_int, c_double, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      real(c_double), value :: t
+      real(c_double), dimension(3), intent(in) :: xyz0
+    end function
   end interface
 
 contains

@@ -124,6 +124,9 @@ def lib():
             "pcmx_ens_tune": [ctypes.c_int],
             "pcmx_ens_set_timing": [ctypes.c_int],
             "pcmx_ens_timing": [dp],
+            "pcmx_write_snapshot": [ctypes.c_char_p, ctypes.c_double, dp],
+            "pcmx_write_record": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long),
+                                  ctypes.c_int, ctypes.c_long],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -222,6 +225,26 @@ def time_step(nsteps, t, dt, ldiag=True):
     tt, dd = ctypes.c_double(t), ctypes.c_double(dt)
     _check(lib().pcmx_time_step(nsteps, ctypes.byref(tt), ctypes.byref(dd), int(ldiag)))
     return tt.value, dd.value
+
+
+def write_snapshot(path, t, xyz0=(-math.pi, -math.pi, -math.pi)):
+    """Writes the device state as a var.dat snapshot (the reference's
+    output_snap layout for one process, src/io_dist.f90:108-199) without
+    copying the f-array to the host.  xyz0: the box origin (start.in xyz0,
+    default -pi in every direction)."""
+    o = (ctypes.c_double * 3)(*[float(v) for v in xyz0])
+    _check(lib().pcmx_write_snapshot(os.fsencode(path), float(t), o))
+
+
+def write_record(path, parts, append=False, max_subrecord=0):
+    """One Fortran unformatted sequential record (gfortran's convention)
+    holding the given float64 arrays back to back; max_subrecord (bytes, 0 =
+    the runtime's limit) exists for tests of the subrecord layout."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in parts]
+    dp = ctypes.POINTER(ctypes.c_double)
+    ptrs = (dp * max(1, len(arrs)))(*[a.ctypes.data_as(dp) for a in arrs])
+    counts = (ctypes.c_long * max(1, len(arrs)))(*[a.size for a in arrs])
+    _check(lib().pcmx_write_record(os.fsencode(path), int(bool(append)), ptrs, counts, len(arrs), int(max_subrecord)))
 
 
 def tune():
