@@ -57,6 +57,10 @@ typedef struct pcmx_params {
   int precision;             /* PCMX_PREC_* */
   int kernel;                /* 0: auto, 1: direct (one thread per cell), 2: z-marching LDS,
                                 3: z-marching LDS with flow/field halves on separate waves */
+  int lupw_uu;               /* hydro_run_pars lupw_uu: upwinded u.grad u (hydro.f90:3662,
+                                sub.f90:7541-7574); the split kernel takes it together with
+                                lupw_lnrho and without entropy, the direct kernel in any case */
+  int lupw_lnrho;            /* density_run_pars lupw_lnrho (density.f90:2772-2773) */
 } pcmx_params;
 
 /* Last error message (thread-local to the calling thread). */
@@ -149,6 +153,41 @@ int pcmx_tune(double* best);
  * stream.  out[0] = mean kernel ms, out[1] = number of kernels timed. */
 int pcmx_kernel_timing(double* out);
 int pcmx_set_timing(int enable);
+
+/* ------------------------------------------------------------------------
+ * Helical forcing (forcing_run_pars iforce='helical').
+ *
+ * pcmx_set_forcing replaces the reference GPU path's forcing update
+ * (src/astaroth/forcing.h:13-29: forcing_pars_hel on the host, parameters
+ * loaded as device uniforms) and its forcing term (src/astaroth/DSL/forcing/
+ * pcstyleforcing.h:15-48, added at the last substep, DSL/solve.ac:64-67);
+ * the CPU path adds the same term after each step (run.f90:575, addforce ->
+ * forcing_hel, forcing.f90:2066-2474).  Default path only: no profiles,
+ * omega_ff=0, force1_scl=1, iforcing_zsym=0, no discrete phases, one force.
+ * After each full step (the end of substep 3 of pcmx_rhs / pcmx_time_step)
+ * the library adds, for j = 0..2,
+ *   f[target+j] += fda[j] * Re((coef1[j] + i coef2[j]) * fx(x) fy(y) fz(z)),
+ *   fx = fact' exp(i (kk[0] k1_ff x + phase)), fy = exp(i kk[1] k1_ff y),
+ *   fz = exp(i kk[2] k1_ff z)                    (fxyz_coefs_hel, :2003-2064)
+ * at the cell centres x = xyz0[0] + (l + 1/2) dx (l = 0..nx-1), where
+ * fact' = fact, or fact*sqrt(dt) of that step when scale_sqrt_dt != 0
+ * (fconst_coefs_hel, :1773-2001, forms fact = force/ffnorm*sqrt(dt); pass
+ * force/ffnorm to have dt applied by the library, as with Courant steps run
+ * by pcmx_time_step).  The parameters stay in force until the next call;
+ * active = 0 switches the forcing off.  target: 0-based slot of the forced
+ * vector's x component, iux (hydro forcing) or iax (lmagnetic_forcing=T). */
+typedef struct pcmx_forcing {
+  int active;
+  int target;
+  int scale_sqrt_dt;
+  int reserved;      /* 0 */
+  double kk[3];      /* wave vector as forcing_pars_hel returns it */
+  double phase, fact;
+  double coef1[3], coef2[3], fda[3];
+  double k1_ff;      /* forcing_run_pars k1_ff */
+  double xyz0[3];    /* box origin (init_pars xyz0) */
+} pcmx_forcing;
+int pcmx_set_forcing(const pcmx_forcing* frc);
 
 /* ------------------------------------------------------------------------
  * Snapshots written from the device.  No counterpart in the reference's

@@ -31,18 +31,33 @@ class Params(ctypes.Structure):
         ("rho0", ctypes.c_double), ("mu0", ctypes.c_double),
         ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
         ("dt", ctypes.c_double),
+        ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
     ]
+
+
+class Forcing(ctypes.Structure):
+    _fields_ = [
+        ("kk", ctypes.c_double * 3), ("phase", ctypes.c_double), ("fact", ctypes.c_double),
+        ("coef1", ctypes.c_double * 3), ("coef2", ctypes.c_double * 3), ("fda", ctypes.c_double * 3),
+        ("k1_ff", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("z0", ctypes.c_double),
+        ("target", ctypes.c_int),
+    ]
+
+
+def make_forcing(kk, phase, fact, coef1, coef2, fda=(1.0, 1.0, 1.0), k1_ff=1.0, xyz0=(-math.pi,) * 3, target=0):
+    return Forcing((ctypes.c_double * 3)(*kk), phase, fact, (ctypes.c_double * 3)(*coef1),
+                   (ctypes.c_double * 3)(*coef2), (ctypes.c_double * 3)(*fda), k1_ff, *xyz0, target)
 
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
                 cs0=1.0, gamma=1.0, cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25,
-                dt=1e-3, L=2 * math.pi):
+                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False):
     """Run parameters with the reference defaults (cdata.f90:235-239,
     eos_idealgas.f90:40-46)."""
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     return Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
-                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt)
+                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho))
 
 
 def build():
@@ -70,8 +85,9 @@ def lib():
         L.pco_derij.argtypes = [dp, pp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.pco_pde.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, ctypes.c_int]
         L.pco_advance.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
+        L.pco_addforce.argtypes = [dp, pp, ctypes.POINTER(Forcing)]
         for name in ("pco_der", "pco_der2", "pco_derij", "pco_fill_ghosts", "pco_pde",
-                     "pco_advance", "pco_eos_constants"):
+                     "pco_advance", "pco_eos_constants", "pco_addforce"):
             getattr(L, name).restype = None
         _lib = L
     return _lib
@@ -146,13 +162,29 @@ def rhs(interior_f, p, want_dt=False, nthreads=0):
     return interior(df).copy(), float(dt1[0]), tuple(diag.tolist())
 
 
-def advance(interior_f, p, nsteps, t=0.0, dt=0.0, diag=False, nthreads=0):
-    """nsteps full RK3 steps.  Returns (f_interior, t, dt, diag_series)."""
+def advance(interior_f, p, nsteps, t=0.0, dt=0.0, diag=False, nthreads=0, forcing=None):
+    """nsteps full RK3 steps.  Returns (f_interior, t, dt, diag_series).
+    forcing (a Forcing) is added after every step, as run.f90:575 calls
+    addforce after time_step."""
     f = with_ghosts(interior_f)
     df = np.zeros_like(f)
     tt = ctypes.c_double(t)
     dd = ctypes.c_double(dt if p.dt == 0.0 else p.dt)
     series = np.zeros((max(nsteps, 1), 3)) if diag else None
-    lib().pco_advance(_ptr(f), _ptr(df), ctypes.byref(p), nsteps, ctypes.byref(tt), ctypes.byref(dd),
-                      _ptr(series) if diag else None, nthreads)
+    if forcing is None:
+        lib().pco_advance(_ptr(f), _ptr(df), ctypes.byref(p), nsteps, ctypes.byref(tt), ctypes.byref(dd),
+                          _ptr(series) if diag else None, nthreads)
+    else:
+        for it in range(nsteps):
+            row = series[it:it + 1] if diag else None
+            lib().pco_advance(_ptr(f), _ptr(df), ctypes.byref(p), 1, ctypes.byref(tt), ctypes.byref(dd),
+                              _ptr(row) if diag else None, nthreads)
+            lib().pco_addforce(_ptr(f), ctypes.byref(p), ctypes.byref(forcing))
     return interior(f).copy(), tt.value, dd.value, series
+
+
+def addforce(interior_f, p, forcing):
+    """The forcing of one step added to an interior state (copy)."""
+    f = with_ghosts(interior_f)
+    lib().pco_addforce(_ptr(f), ctypes.byref(p), ctypes.byref(forcing))
+    return interior(f).copy()

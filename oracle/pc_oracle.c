@@ -166,6 +166,38 @@ static void der2(const geom* g, const double* f, int m, int n, int k, int j, dou
   }
 }
 
+/* der6_main with UPWIND=.true. (deriv.f90:936-1001): fac=(1/60)*dx_1,
+ * df = fac*(-20 f + 15 (f+1 + f-1) - 6 (f+2 + f-2) + (f+3 + f-3)). */
+static void der6_upw(const geom* g, const double* f, int m, int n, int k, int j, double* df) {
+  const int nx = g->nx;
+  long s;
+  double fac;
+  if (j == 0) { s = 1; fac = (1.0 / 60) * g->dx_1; }
+  else if (j == 1) { s = g->mx; fac = (1.0 / 60) * g->dy_1; }
+  else { s = (long)g->mx * g->my; fac = (1. / 60.) * g->dz_1; }
+  const double* r = &F(g, f, NG, m, n, k);
+  for (int l = 0; l < nx; ++l) {
+    const double* c = r + l;
+    df[l] = fac * (-(20.0 * c[0]) + 15.0 * (c[s] + c[-s]) - 6.0 * (c[2 * s] + c[-2 * s]) + (c[3 * s] + c[-3 * s]));
+  }
+}
+
+/* doupwind (sub.f90:7541-7574) through calc_del6_for_upwind (:7491-7539):
+ * ugradf -= sum_j |u_j| * der6_upw_j(f_k), the sum over j in order. */
+static void doupwind(const geom* g, const double* f, int m, int n, int k, const double* uu, double* ugradf,
+                     double* tmp) {
+  const int nx = g->nx;
+  double* d = tmp;  /* 3*nx */
+  for (int j = 0; j < 3; ++j) {
+    der6_upw(g, f, m, n, k, j, d + (long)j * nx);
+    for (int l = 0; l < nx; ++l) d[(long)j * nx + l] = fabs(uu[(long)j * nx + l]) * d[(long)j * nx + l];
+  }
+  for (int l = 0; l < nx; ++l) {
+    const double up = d[l] + d[nx + l] + d[2 * nx + l];  /* sum(del6f,2) */
+    ugradf[l] = ugradf[l] - up;
+  }
+}
+
 /* derij_main bidiagonal branch, deriv.f90:1400-1452.  (i,j) unordered,
  * i != j.  The operand order inside each bracket follows the reference,
  * which differs between the xy/xz and the yz branches. */
@@ -216,7 +248,8 @@ enum {
   P_MAXADVEC = P_ADVVA2 + 1, P_ADVEC2 = P_MAXADVEC + 1, P_ADVECCS2 = P_ADVEC2 + 1,
   P_MAXDIFFUS = P_ADVECCS2 + 1, P_DT1MAX = P_MAXDIFFUS + 1,
   P_TMP = P_DT1MAX + 1, P_TMP2 = P_TMP + 1, P_FIJJ = P_TMP2 + 1, P_FJJI = P_FIJJ + 9,
-  P_COUNT = P_FJJI + 9
+  P_UPW = P_FJJI + 9 /* scratch of the upwind correction (3) */,
+  P_COUNT = P_UPW + 3
 };
 
 typedef struct {
@@ -308,6 +341,9 @@ static void calc_pencils(const ctx* c, const double* f, int m, int n, pencils* p
   }
   /* ugu: u_dot_grad_vec (sub.f90:3482-3534) -> dot_mn(uu, uij(:,j,:)) */
   for (int j = 0; j < 3; ++j) dot_mn(nx, PC(pc, P_U), nx, UIJ(pc, j, 0), nx, PC(pc, P_UGU + j));
+  /* lupw_uu: u_dot_grad_scl(...,UPWIND=lupw_uu) per component (sub.f90:3507, :3680) */
+  if (p->lupw_uu)
+    for (int j = 0; j < 3; ++j) doupwind(g, f, m, n, j, PC(pc, P_U), PC(pc, P_UGU + j), PC(pc, P_UPW));
   /* divu: div_mn, sub.f90:1981-2012 */
   for (int l = 0; l < nx; ++l) PC(pc, P_DIVU)[l] = UIJ(pc, 0, 0)[l] + UIJ(pc, 1, 1)[l] + UIJ(pc, 2, 2)[l];
   /* sij: traceless_strain, sub.f90:8922-8989 */
@@ -345,6 +381,8 @@ static void calc_pencils(const ctx* c, const double* f, int m, int n, pencils* p
   for (int l = 0; l < nx; ++l) PC(pc, P_RHO1)[l] = exp(-PC(pc, P_LNRHO)[l]);
   for (int j = 0; j < 3; ++j) der(g, f, m, n, 3, j, PC(pc, P_GLNRHO + j));
   dot_mn(nx, PC(pc, P_U), nx, PC(pc, P_GLNRHO), nx, PC(pc, P_UGLNRHO));
+  /* lupw_lnrho: u_dot_grad(f,ilnrho,glnrho,uu,uglnrho,UPWIND=lupw_lnrho) (density.f90:2772-2773) */
+  if (p->lupw_lnrho) doupwind(g, f, m, n, 3, PC(pc, P_U), PC(pc, P_UGLNRHO), PC(pc, P_UPW));
   if (p->lentropy) {
     /* del2(f,ilnrho): del2_main, sub.f90:2304-2336 */
     double* t = PC(pc, P_TMP);
@@ -755,4 +793,51 @@ void pco_der2(const double* f, const pco_params* p, int k, int j, double* out) {
 }
 void pco_derij(const double* f, const pco_params* p, int k, int i, int j, double* out) {
   for_all_pencils(f, p, out, derij, k, i, j);
+}
+
+/* ------------------------------------------------------------------ */
+/* addforce, helical default path: see pc_oracle.h.                     */
+void pco_addforce(double* f, const pco_params* p, const pco_forcing* fo) {
+  const geom gg = make_geom(p);
+  const geom* g = &gg;
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const double dx = p->Lx / p->nx, dy = p->Ly / p->ny, dz = p->Lz / p->nz;
+  double* fxr = malloc(sizeof(double) * 2 * (nx + ny + nz));
+  double *fxi = fxr + nx, *fyr = fxi + nx, *fyi = fyr + ny, *fzr = fyi + ny, *fzi = fzr + nz;
+  /* fxyz_coefs_hel: fx=exp(cmplx(0.,kk(1)*k1_ff*x+phase))*fact, fy, fz */
+  for (int l = 0; l < nx; ++l) {
+    const double x = fo->x0 + (l + 0.5) * dx;
+    const double th = fo->kk[0] * fo->k1_ff * x + fo->phase;
+    fxr[l] = cos(th) * fo->fact;
+    fxi[l] = sin(th) * fo->fact;
+  }
+  for (int m = 0; m < ny; ++m) {
+    const double y = fo->y0 + (m + 0.5) * dy;
+    const double th = fo->kk[1] * fo->k1_ff * y;
+    fyr[m] = cos(th);
+    fyi[m] = sin(th);
+  }
+  for (int n = 0; n < nz; ++n) {
+    const double z = fo->z0 + (n + 0.5) * dz;
+    const double th = fo->kk[2] * fo->k1_ff * z;
+    fzr[n] = cos(th);
+    fzi[n] = sin(th);
+  }
+#pragma omp parallel for schedule(static)
+  for (int n = 0; n < nz; ++n)
+    for (int m = 0; m < ny; ++m)
+      for (int l = 0; l < nx; ++l) {
+        /* fxyz = fx(l)*fy(m)*fz(n), complex products left to right */
+        const double ar = fxr[l] * fyr[m] - fxi[l] * fyi[m];
+        const double ai = fxr[l] * fyi[m] + fxi[l] * fyr[m];
+        const double br = ar * fzr[n] - ai * fzi[n];
+        const double bi = ar * fzi[n] + ai * fzr[n];
+        for (int j = 0; j < 3; ++j) {
+          /* forcing_rhs = force_ampl*fda(j)*cos(omega_ff*t)*real(cmplx(coef1(j),coef2(j))*fxyz) */
+          const double re = fo->coef1[j] * br - fo->coef2[j] * bi;
+          const double rhs = fo->fda[j] * re;
+          F(g, f, NG + l, NG + m, NG + n, fo->target + j) += rhs;
+        }
+      }
+  free(fxr);
 }

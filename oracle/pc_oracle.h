@@ -45,7 +45,27 @@ typedef struct {
   double cs0, gamma, cp, rho0, mu0;
   double cdt, cdtv;        /* Courant factors (cdata.f90:235-239) */
   double dt;               /* fixed dt; 0 => Courant-limited (timestep.f90:67) */
+  int lupw_uu;             /* hydro_run_pars lupw_uu: upwinded u.grad u (hydro.f90:3662) */
+  int lupw_lnrho;          /* density_run_pars lupw_lnrho: upwinded u.grad lnrho (density.f90:2772-2773) */
 } pco_params;
+
+/* Helical forcing of one step, as forcing_pars_hel returns it
+ * (forcing.f90:1707-2001) plus the grid origin and the forced variable. */
+typedef struct {
+  double kk[3], phase, fact;
+  double coef1[3], coef2[3], fda[3];
+  double k1_ff;
+  double x0, y0, z0;       /* box origin (start.in xyz0) */
+  int target;              /* first forced slot: 0 (uu) or iax (lmagnetic_forcing) */
+} pco_forcing;
+
+/* addforce with iforce='helical' on its default path (forcing.f90:1307-1395,
+ * 2066-2474: unit profiles, omega_ff=0, dtforce=0, force1_scl=1) and
+ * fxyz_coefs_hel (:2003-2064, iforcing_zsym=0, no discrete phases):
+ * f(l,m,n,target+j) += fda(j)*real(cmplx(coef1(j),coef2(j))*fx(l)*fy(m)*fz(n)),
+ * fx = fact*exp(i*(kk(1)*k1_ff*x+phase)), fy = exp(i*kk(2)*k1_ff*y),
+ * fz = exp(i*kk(3)*k1_ff*z), x(l) = x0 + (l + 1/2)*dx on the interior. */
+void pco_addforce(double* f, const pco_params* p, const pco_forcing* fo);
 
 /* Variable slot helpers (0-based). */
 int pco_nvar(const pco_params* p);

@@ -20,7 +20,7 @@ struct Slots {
   static constexpr int IAX = MAG ? (ENT ? 5 : 4) : -1;
 };
 
-template <typename T, bool MAG, bool ENT, bool DT>
+template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0>
 __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
   using S = Slots<ENT, MAG>;
   constexpr int NV = S::NV;
@@ -99,6 +99,20 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
       d.glnrho[i] = d1(3, i);
     }
     d.lnrho = (T)fc[3];
+    if constexpr (UPW != 0) {
+      // upwind corrections: sum over j of |u_j| D6_j f, the sum in order x, y, z
+      // (calc_del6_for_upwind, sub.f90:7508-7537)
+      auto d6 = [&](int v, int dir) -> T {
+        return der6_upwind<T>((T)P.fac1[dir], at(v, dir, 0), at(v, dir, 1) + at(v, dir, -1),
+                              at(v, dir, 2) + at(v, dir, -2), at(v, dir, 3) + at(v, dir, -3));
+      };
+      const T ua[3] = {(T)fabs(fc[0]), (T)fabs(fc[1]), (T)fabs(fc[2])};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if (((v < 3) ? (UPW & UPW_UU) : (UPW & UPW_LNRHO)) == 0) continue;
+        d.upw[v] = ua[0] * d6(v, 0) + ua[1] * d6(v, 1) + ua[2] * d6(v, 2);
+      }
+    }
     if constexpr (ENT) {
       d.ss = (T)fc[S::ISS];
 #pragma unroll
@@ -130,7 +144,7 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) w[v] = a.first ? 0.0 : a.alpha * a.w[v * nxyz + idx];
     double dt1 = 0.0, u2 = 0.0, b2 = 0.0;
-    rhs_point<T, MAG, ENT, DT>(d, P, w, S::ISS, S::IAX, &dt1, &u2, &b2);
+    rhs_point<T, MAG, ENT, DT, UPW>(d, P, w, S::ISS, S::IAX, &dt1, &u2, &b2);
 
     if (a.mode == MODE_UPDATE) {
       const double dtb = a.d_dt ? a.d_dt[0] * a.beta : a.dtb;

@@ -10,39 +10,61 @@
 
 namespace pcmx {
 
-template <typename T, bool MAG, bool ENT, bool DT>
+template <typename T, bool MAG, bool ENT, bool DT, int UPW>
 void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
   dim3 blk(64, 4, 1), grd((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
-  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT>), grd, blk, 0, s, a, P);
+  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT, UPW>), grd, blk, 0, s, a, P);
 }
 
-template <typename T, bool MAG, bool ENT, class TL>
+template <typename T, bool MAG, bool ENT, class TL, int UPW>
 bool launch_split(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
   switch (a.stage) {
-    case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1, TL>(a, P, zp, s); return true;
-    case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2, TL>(a, P, zp, s); return true;
-    case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3, TL>(a, P, zp, s); return true;
-    case ST_W: launch_zsplit<T, MAG, ENT, ST_W, TL>(a, P, zp, s); return true;
-    case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D, TL>(a, P, zp, s); return true;
-    case ST_DT: launch_zsplit<T, MAG, ENT, ST_DT, TL>(a, P, zp, s); return true;
+    case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1, TL, UPW>(a, P, zp, s); return true;
+    case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2, TL, UPW>(a, P, zp, s); return true;
+    case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3, TL, UPW>(a, P, zp, s); return true;
+    case ST_W: launch_zsplit<T, MAG, ENT, ST_W, TL, UPW>(a, P, zp, s); return true;
+    case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D, TL, UPW>(a, P, zp, s); return true;
+    case ST_DT: launch_zsplit<T, MAG, ENT, ST_DT, TL, UPW>(a, P, zp, s); return true;
     default: return false;
   }
 }
 
+template <typename T, bool MAG, bool ENT, int UPW>
+void launch_direct_dt(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  if (c.dt) launch_direct<T, MAG, ENT, true, UPW>(a, P, s);
+  else launch_direct<T, MAG, ENT, false, UPW>(a, P, s);
+}
+
+// Upwinding: every mask in the direct kernel; both corrections (lupw_uu and
+// lupw_lnrho, the published configuration) without entropy in the split
+// kernel; none in the first z-marching kernel.
 template <typename T, bool MAG, bool ENT>
 bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
   if (c.kernel == 3) {
-    if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16>(a, P, c.zp, s);
-    return launch_split<T, MAG, ENT, ZT32x8>(a, P, c.zp, s);
+    if (c.upw == 0) {
+      if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16, 0>(a, P, c.zp, s);
+      return launch_split<T, MAG, ENT, ZT32x8, 0>(a, P, c.zp, s);
+    }
+    if constexpr (!ENT) {
+      if (c.upw != (UPW_UU | UPW_LNRHO)) return false;
+      if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16, UPW_UU | UPW_LNRHO>(a, P, c.zp, s);
+      return launch_split<T, MAG, ENT, ZT32x8, UPW_UU | UPW_LNRHO>(a, P, c.zp, s);
+    }
+    return false;
   }
   if (c.kernel == 2) {
+    if (c.upw != 0) return false;
     if (c.dt) launch_zmarch<T, MAG, ENT, true>(a, P, c.zp, s);
     else launch_zmarch<T, MAG, ENT, false>(a, P, c.zp, s);
     return true;
   }
-  if (c.dt) launch_direct<T, MAG, ENT, true>(a, P, s);
-  else launch_direct<T, MAG, ENT, false>(a, P, s);
-  return true;
+  switch (c.upw) {
+    case 0: launch_direct_dt<T, MAG, ENT, 0>(c, a, P, s); return true;
+    case UPW_UU: launch_direct_dt<T, MAG, ENT, UPW_UU>(c, a, P, s); return true;
+    case UPW_LNRHO: launch_direct_dt<T, MAG, ENT, UPW_LNRHO>(c, a, P, s); return true;
+    case UPW_UU | UPW_LNRHO: launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_LNRHO>(c, a, P, s); return true;
+    default: return false;
+  }
 }
 
 template <typename T>

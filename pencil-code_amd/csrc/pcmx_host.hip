@@ -65,6 +65,12 @@ int check_params(const pcmx_params* p) {
   if (ent && p->gamma == 1.0) return fail(PCMX_ERR_ARG, "entropy needs gamma /= 1");
   if (p->cdt <= 0 || p->cdtv <= 0) return fail(PCMX_ERR_ARG, "cdt and cdtv must be positive");
   if (p->kernel < 0 || p->kernel > 3) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
+  if ((p->lupw_uu != 0 && p->lupw_uu != 1) || (p->lupw_lnrho != 0 && p->lupw_lnrho != 1))
+    return fail(PCMX_ERR_ARG, "lupw_uu and lupw_lnrho are 0 or 1");
+  const bool upw = p->lupw_uu || p->lupw_lnrho;
+  if (upw && p->kernel == 2) return fail(PCMX_ERR_ARG, "upwinding is not provided by kernel 2");
+  if (upw && p->kernel == 3 && (ent || p->lupw_uu != p->lupw_lnrho))
+    return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu and lnrho together and without entropy; use kernel 1");
   return PCMX_OK;
 }
 
@@ -139,6 +145,9 @@ struct Engine {
   // pinned chunks of the device -> file snapshot path
   double* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  // helical forcing added after each step (pcmx_set_forcing)
+  pcmx_forcing frc{};
+  double2* ftab = nullptr;    // nx + ny + nz unit phasors
 };
 
 Engine* g = nullptr;
@@ -156,6 +165,7 @@ LaunchCfg cfg_of(const Engine& e, bool dt) {
   c.ent = e.ent;
   c.dt = dt;
   c.zp = e.zplan;
+  c.upw = (e.par.lupw_uu ? UPW_UU : 0) | (e.par.lupw_lnrho ? UPW_LNRHO : 0);
   return c;
 }
 
@@ -281,9 +291,30 @@ int substep_deferred(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0
   return PCMX_OK;
 }
 
+// The forcing of the step just completed, added to the state (addforce after
+// time_step, run.f90:575).  dt is the step's dt, known on the host by now.
+int add_forcing(Engine& e, double dt) {
+  const pcmx_forcing& fr = e.frc;
+  ForceArgs fa{};
+  for (int j = 0; j < 3; ++j) {
+    fa.c1[j] = fr.coef1[j];
+    fa.c2[j] = fr.coef2[j];
+    fa.fda[j] = fr.fda[j];
+  }
+  fa.fact = fr.scale_sqrt_dt ? fr.fact * std::sqrt(dt) : fr.fact;  // force/ffnorm*sqrt(dt), forcing.f90:1958
+  fa.target = fr.target;
+  const dim3 grid((e.par.nx + 63) / 64, (e.par.ny + 3) / 4, e.par.nz);
+  hipLaunchKernelGGL(k_add_forcing, grid, dim3(256), 0, e.stream, e.f[e.cur], e.ftab, fa, e.par.nx, e.par.ny,
+                     e.par.nz);
+  HIPCHECK(hipGetLastError());
+  return PCMX_OK;
+}
+
 int run_substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
-  if (e.kernel == 3 && e.par.ldt) return substep_deferred(e, s, dt_host, diag, ev0, ev1);
-  return substep(e, s, dt_host, diag, ev0, ev1);
+  const int rc = (e.kernel == 3 && e.par.ldt) ? substep_deferred(e, s, dt_host, diag, ev0, ev1)
+                                              : substep(e, s, dt_host, diag, ev0, ev1);
+  if (rc || s != 2 || !e.frc.active) return rc;
+  return add_forcing(e, dt_host);
 }
 
 // The state as the reference would hold it: after substep 1 of kernel 3,
@@ -302,6 +333,7 @@ void free_engine(Engine* e) {
   if (!e) return;
   for (double* p : {e->f[0], e->f[1], e->w, e->w2, e->partials, e->results, e->staging})
     if (p) (void)hipFree(p);
+  if (e->ftab) (void)hipFree(e->ftab);
   for (int k = 0; k < 2; ++k) {
     if (e->pin[k]) (void)hipHostFree(e->pin[k]);
     if (e->pin_ev[k]) (void)hipEventDestroy(e->pin_ev[k]);
@@ -392,7 +424,9 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   // kernel choice: the z-marching LDS kernels need whole tiles in x and y
   int kern = par->kernel;
   const int tile = (par->nz >= 4) ? split_tile(par->nx, par->ny) : -1;
-  if (kern == 0) kern = tile >= 0 ? 3 : 1;
+  // upwinding of uu alone, of lnrho alone, or with entropy: the direct kernel
+  const bool upw_split = (par->lupw_uu == par->lupw_lnrho) && !(par->lupw_uu && par->iss >= 0);
+  if (kern == 0) kern = (tile >= 0 && upw_split) ? 3 : 1;
   if (kern == 2 && !zmarch_supported(*par)) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
@@ -788,6 +822,51 @@ int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag) {
     e.timing_n = 3L * nsteps;
     for (auto& ev : evs) (void)hipEventDestroy(ev);
   }
+  return PCMX_OK;
+}
+
+// ------------------------------------------------------------------ forcing
+
+int pcmx_set_forcing(const pcmx_forcing* frc) {
+  int rc = require();
+  if (rc) return rc;
+  Engine& e = *g;
+  if (!frc) return fail(PCMX_ERR_ARG, "frc is NULL");
+  if (!frc->active) {
+    e.frc = pcmx_forcing{};
+    return PCMX_OK;
+  }
+  const pcmx_params& p = e.par;
+  if (frc->active != 1 || frc->reserved != 0 || (frc->scale_sqrt_dt != 0 && frc->scale_sqrt_dt != 1))
+    return fail(PCMX_ERR_ARG, "pcmx_forcing: active and scale_sqrt_dt are 0 or 1, reserved is 0");
+  if (frc->target != p.iux && !(e.mag && frc->target == p.iax))
+    return fail(PCMX_ERR_ARG, "pcmx_forcing: target %d is neither iux (%d) nor iax (%d)", frc->target, p.iux,
+                e.mag ? p.iax : -1);
+  const double* vals[] = {frc->kk, frc->coef1, frc->coef2, frc->fda, frc->xyz0};
+  for (const double* v : vals)
+    for (int j = 0; j < 3; ++j)
+      if (!std::isfinite(v[j])) return fail(PCMX_ERR_ARG, "pcmx_forcing: non-finite parameter");
+  if (!std::isfinite(frc->phase) || !std::isfinite(frc->fact) || !std::isfinite(frc->k1_ff))
+    return fail(PCMX_ERR_ARG, "pcmx_forcing: non-finite parameter");
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const double d[3] = {p.dx, p.dy, p.dz};
+  std::vector<double2> tab((size_t)p.nx + p.ny + p.nz);
+  size_t o = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int l = 0; l < n[a]; ++l) {
+      // cell centres of the periodic grid (grid.f90:110-216); exp(cmplx(0., kk*k1_ff*x + phase))
+      double c;
+      {
+#pragma clang fp contract(off)
+        c = frc->xyz0[a] + (l + 0.5) * d[a];
+      }
+      const double th = frc->kk[a] * frc->k1_ff * c + (a == 0 ? frc->phase : 0.0);
+      tab[o++] = make_double2(std::cos(th), std::sin(th));
+    }
+  if (!e.ftab) HIPCHECK(hipMalloc(&e.ftab, tab.size() * sizeof(double2)));
+  HIPCHECK(hipMemcpyAsync(e.ftab, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice, e.stream));
+  HIPCHECK(hipStreamSynchronize(e.stream));
+  e.frc = *frc;
   return PCMX_OK;
 }
 

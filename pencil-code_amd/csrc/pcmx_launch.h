@@ -24,6 +24,7 @@ struct LaunchCfg {
   bool mag, ent;
   bool dt;      // kernels 1 and 2: evaluate the Courant terms
   ZPlan zp;     // kernels 2 and 3: the grid of tile columns
+  int upw = 0;  // upwinding mask (UPW_UU | UPW_LNRHO)
 };
 
 struct EnsBox;

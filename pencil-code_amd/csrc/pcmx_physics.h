@@ -57,7 +57,23 @@ struct Derivs {
   T del2lnrho, del2ss;
   T d2a[3][3];        // D2_j A_i                      (gij_etc d2A(j,j,i))
   T mxa[3][3];
+  // upwind corrections sum_j |u_j| D6_j f of u_x, u_y, u_z, lnrho (doupwind,
+  // sub.f90:7491-7574; D6_j = der6_main with UPWIND, deriv.f90:936-1001),
+  // filled only with upwinding
+  T upw[4];
 };
+
+// Upwinding masks (compile time): UPW_UU for lupw_uu (hydro.f90:3662),
+// UPW_LNRHO for lupw_lnrho (density.f90:2772-2773).
+constexpr int UPW_UU = 1, UPW_LNRHO = 2;
+
+// der6_main with UPWIND=.true. along one direction from the symmetric sums
+// s_k = f(+k) + f(-k): fac*(-20 f0 + 15 s1 - 6 s2 + s3) with fac = dx_1/60,
+// the operand order of deriv.f90:990-993.
+template <typename T>
+__device__ __forceinline__ T der6_upwind(T fac, T c0, T s1, T s2, T s3) {
+  return fac * (-((T)20.0 * c0) + (T)15.0 * s1 - (T)6.0 * s2 + s3);
+}
 
 // Adds the second derivatives of component i of uu to del2u and gdivu.
 // mixed[] holds D_xy, D_xz, D_yz of u_i (entries not used by graddiv are
@@ -180,7 +196,7 @@ __device__ __forceinline__ void field_point(const Derivs<T>& d, T rho1, const Ph
 // argument so that both halves use the same value.  Without entropy the EOS
 // is isothermal (gamma = 1) or, with POLY, polytropic (noentropy.f90:94-103);
 // it is a compile-time choice so that no branch splits the algebra.
-template <typename T, bool ENT, bool DT, bool POLY>
+template <typename T, bool ENT, bool DT, bool POLY, int UPW = 0>
 __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const Phys& P, double* wf,
                                                FlowKeep<T>& k, double* u2_out) {
   // ---- hydro pencils (hydro.f90:3925-4000; sub.f90 helpers) ----
@@ -201,12 +217,17 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
   T ugu[3];                                                             // u_dot_grad_vec
 #pragma unroll
   for (int j = 0; j < 3; ++j) ugu[j] = u0 * d.uij[j][0] + u1 * d.uij[j][1] + u2v * d.uij[j][2];
+  if constexpr ((UPW & UPW_UU) != 0) {  // doupwind: ugradf = ugradf - del6f_upwind
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ugu[j] = ugu[j] - d.upw[j];
+  }
   const T* del2u = d.del2u;                                             // del2v_etc
   const T* gdivu = d.gdivu;
 
   // ---- density pencils (density.f90:2727-2811) ----
   const T lnrho = d.lnrho;
-  const T uglnrho = u0 * d.glnrho[0] + u1 * d.glnrho[1] + u2v * d.glnrho[2];
+  T uglnrho = u0 * d.glnrho[0] + u1 * d.glnrho[1] + u2v * d.glnrho[2];
+  if constexpr ((UPW & UPW_LNRHO) != 0) uglnrho = uglnrho - d.upw[3];
   T sglnrho[3];                                                         // multmv_mn
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -301,7 +322,7 @@ __device__ __forceinline__ void flow_point_post(const FlowKeep<T>& k, const Fiel
 
 // Whole RHS of one cell: adds it to w[] (already alpha-scaled; slots 0..2 uu,
 // 3 lnrho, [iss], [iax..iax+2]) and returns dt1_max_loc (when DT), u2 and b2.
-template <typename T, bool MAG, bool ENT, bool DT>
+template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0>
 __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, double* w, int iss,
                                           int iax, double* dt1, double* u2_out, double* b2_out) {
   const T rho1 = pexp<T>(-d.lnrho);
@@ -313,8 +334,8 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
   for (int v = 0; v < 4; ++v) wf[v] = w[v];
   if constexpr (ENT) wf[4] = w[iss];
   FlowKeep<T> k;
-  if (!ENT && P.polytropic) flow_point_pre<T, ENT, DT, true>(d, rho1, P, wf, k, u2_out);
-  else flow_point_pre<T, ENT, DT, false>(d, rho1, P, wf, k, u2_out);
+  if (!ENT && P.polytropic) flow_point_pre<T, ENT, DT, true, UPW>(d, rho1, P, wf, k, u2_out);
+  else flow_point_pre<T, ENT, DT, false, UPW>(d, rho1, P, wf, k, u2_out);
   flow_point_post<T, MAG, ENT, DT>(k, fo, P, wf, dt1);
 #pragma unroll
   for (int v = 0; v < 4; ++v) w[v] = wf[v];

@@ -211,7 +211,7 @@ constexpr int zs_xch_elems() {
 // Marches one tile column (x0, y0, z0 .. z0+lz-1) of one box through the
 // RK stage ST.  ring and xch are the block's LDS buffers; the column's
 // reduction terms are added to red (all threads of the block take part).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS, int RING>
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS, int RING, int UPW = 0>
 __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int x0, int y0, int z0, int lz,
                                               T* ring, T* xch, double (&red)[NRED]) {
   using S = Slots<ENT, MAG>;
@@ -416,6 +416,15 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   const T C0 = (T)PCMX_D2C0, C1 = (T)PCMX_D2C1, C2 = (T)PCMX_D2C2, C3 = (T)PCMX_D2C3;
   const double dtb = a.d_dt ? a.d_dt[0] * a.beta : a.dtb;
 
+  // |u| of the own cell for the upwind corrections of the flow half (set at
+  // the start of each plane when UPW != 0)
+  T ua[3] = {(T)0, (T)0, (T)0};
+  // does variable V take an upwind correction?
+  auto upwinded = [](auto v_c) {
+    constexpr int V = decltype(v_c)::value;
+    return std::bool_constant<(V < 3 && (UPW & UPW_UU) != 0) || (V == 3 && (UPW & UPW_LNRHO) != 0)>{};
+  };
+
   // Stencils at plane q of the variables of one staged plane (PT = T2: a
   // pair, components 0 and 1 are variables VA and VB with group-local
   // indices IA_ and IB_; PT = T: a single plane, VB = -1).  Every tap is one
@@ -452,6 +461,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     const PT c0 = p0[0];
     T rdx[2] = {}, rdy[2] = {}, rdz[2] = {}, rxx[2] = {}, ryy[2] = {}, rzz[2] = {};
     T rxy[2] = {}, rxz[2] = {}, ryz[2] = {};
+    T upx[2] = {}, upy[2] = {}, upz[2] = {};  // |u_j| D6_j f per direction
     if constexpr (EMIT) {
       // own column: z derivatives (der_main, der2_main; deriv.f90:124-162, 497-550)
       const PT zp1 = p1[0], zp2 = p2[0], zp3 = p3[0];
@@ -462,6 +472,9 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
         if constexpr (N::d2(V))
           rzz[C] = fz2 * (C0 * pick<C>(c0) + C1 * (pick<C>(zp1) + zm1) + C2 * (pick<C>(zp2) + zm2) +
                           C3 * (pick<C>(zp3) + zm3));
+        if constexpr (decltype(upwinded(v_c))::value)
+          upz[C] = ua[2] * der6_upwind<T>(fz1, pick<C>(c0), pick<C>(zp1) + zm1, pick<C>(zp2) + zm2,
+                                          pick<C>(zp3) + zm3);
       });
       if constexpr (XZ) {  // forward half of the xz bidiagonal
         const PT a1 = p1[1], b1 = p1[-1], a2 = p2[2], b2 = p2[-2], a3 = p3[3], b3 = p3[-3];
@@ -500,6 +513,9 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
           if constexpr (N::d2(V))
             rxx[C] = fx2 * (C0 * pick<C>(c0) + C1 * (pick<C>(xp1) + pick<C>(xm1)) +
                             C2 * (pick<C>(xp2) + pick<C>(xm2)) + C3 * (pick<C>(xp3) + pick<C>(xm3)));
+          if constexpr (decltype(upwinded(v_c))::value)
+            upx[C] = ua[0] * der6_upwind<T>(fx1, pick<C>(c0), pick<C>(xp1) + pick<C>(xm1),
+                                            pick<C>(xp2) + pick<C>(xm2), pick<C>(xp3) + pick<C>(xm3));
         }
         if constexpr (N::xz(V)) {
           T* m = cxz[N::ixz(V) & 1];
@@ -519,6 +535,9 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
           if constexpr (N::d2(V))
             ryy[C] = fy2 * (C0 * pick<C>(c0) + C1 * (pick<C>(yp1) + pick<C>(ym1)) +
                             C2 * (pick<C>(yp2) + pick<C>(ym2)) + C3 * (pick<C>(yp3) + pick<C>(ym3)));
+          if constexpr (decltype(upwinded(v_c))::value)
+            upy[C] = ua[1] * der6_upwind<T>(fy1, pick<C>(c0), pick<C>(yp1) + pick<C>(ym1),
+                                            pick<C>(yp2) + pick<C>(ym2), pick<C>(yp3) + pick<C>(ym3));
         }
         if constexpr (N::yz(V)) {
           T* m = cyz[N::iyz(V) & 1];
@@ -548,6 +567,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
       each([&](auto c_c, auto v_c, auto) {
         constexpr int C = decltype(c_c)::value, V = decltype(v_c)::value;
         file_derivs<T, MAG, ENT>(d, V, rdx[C], rdy[C], rdz[C], rxx[C], ryy[C], rzz[C], rxy[C], rxz[C], ryz[C]);
+        if constexpr (decltype(upwinded(v_c))::value) d.upw[V] = upx[C] + upy[C] + upz[C];  // x, y, z in order
       });
     }
     // the own column's z window moves up one plane
@@ -616,6 +636,20 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
     }
     if constexpr (GRP == 0) {
       Derivs<T> d;
+      if constexpr (EMIT && UPW != 0) {
+        // |u| of the own cell (the FP64 values in the mixed form, whose ring
+        // holds offsets from the box means)
+        if constexpr (MIXED) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ua[j] = (T)fabs(fcur[j]);
+        } else if constexpr (PAIRS) {
+          const T2 u01 = pair_at(L0, 0)[c], u2l = pair_at(L0, 1)[c];
+          ua[0] = fabs(u01.x), ua[1] = fabs(u01.y), ua[2] = fabs(u2l.x);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) ua[j] = fabs(single_at(L0, j)[c]);
+        }
+      }
       T cv[5];
       if constexpr (PAIRS) {
         const T2 c01 = dpair(IC<0>{}, IC<0>{}, IC<0>{}, IC<1>{}, IC<1>{}, d, sink);
@@ -639,7 +673,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 #pragma unroll
         for (int i = 0; i < NF; ++i) wf[i] = G::READW ? a.alpha * wcur[i] : 0.0;
         const T rho1 = pexp<T>(-d.lnrho);
-        flow_point_pre<T, ENT, DT, POLY>(d, rho1, P, wf, keep, &u2c);
+        flow_point_pre<T, ENT, DT, POLY, UPW>(d, rho1, P, wf, keep, &u2c);
       }
     } else if constexpr (MAG) {
       Derivs<T> d;
@@ -766,7 +800,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 // One box: block b marches tile column wid(b) (XCD-aware: consecutive work
 // items, neighbouring tiles that share halo rims, go to the same XCD, whose
 // L2 then serves the rims).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL>
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, int UPW = 0>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
   constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, false>();
   constexpr int RING = zs_ring_slots<T, MAG, ENT, ST, TL, PAIRS>();
@@ -781,8 +815,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   double red[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz, zp.lz, ring,
-                                                       xch, red);
+  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING, UPW>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz,
+                                                            zp.lz, ring, xch, red);
   if constexpr (StageT<ST>::DIAG) {
     if (a.partials) block_reduce_store<zs_threads<MAG>()>(red, a.partials, (long)blockIdx.x);
   }
@@ -834,13 +868,13 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsB
   }
 }
 
-template <typename T, bool MAG, bool ENT, int ST, class TL>
+template <typename T, bool MAG, bool ENT, int ST, class TL, int UPW = 0>
 void launch_zsplit(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
   if (!ENT && P.polytropic)
-    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST, TL>), dim3((unsigned)zp.nblocks),
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST, TL, UPW>), dim3((unsigned)zp.nblocks),
                        dim3(zs_threads<MAG>()), 0, s, a, P, zp);
   else
-    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST, TL>), dim3((unsigned)zp.nblocks),
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST, TL, UPW>), dim3((unsigned)zp.nblocks),
                        dim3(zs_threads<MAG>()), 0, s, a, P, zp);
 }
 

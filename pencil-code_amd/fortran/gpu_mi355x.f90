@@ -66,11 +66,15 @@ contains
 !
       use SharedVariables, only: get_shared_variable
       use EquationOfState, only: rho0
+      use Hydro, only: hydro_pushpars2c => pushpars2c
+      use Density, only: density_pushpars2c => pushpars2c
 !
       real, dimension(:,:,:,:), intent(in) :: f
 !
       type(pcmx_params) :: par
       real, pointer :: nu_p, eta_p, chi_p, cs20_p, gamma_p, cp_p
+      integer(KIND=ikind8), dimension(200) :: p_par
+      logical, pointer :: lupw
       character(len=512) :: str
 !
       str=''
@@ -113,6 +117,20 @@ contains
       par%ldt = merge(1, 0, ldt)
       par%precision = gpu_precision
       par%kernel = gpu_kernel
+!
+!  Upwinding flags are private to Hydro and Density; their addresses reach
+!  the GPU side through the modules' pushpars2c lists, as in the reference's
+!  GPU path (hydro.f90:9367 p_par(2) = lupw_uu, density.f90:4593 p_par(5) =
+!  lupw_lnrho).
+!
+      p_par = 0
+      call hydro_pushpars2c(p_par)
+      call c_f_pointer(transfer(p_par(2), c_null_ptr), lupw)
+      par%lupw_uu = merge(1, 0, lupw)
+      p_par = 0
+      call density_pushpars2c(p_par)
+      call c_f_pointer(transfer(p_par(5), c_null_ptr), lupw)
+      par%lupw_lnrho = merge(1, 0, lupw)
 !
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)

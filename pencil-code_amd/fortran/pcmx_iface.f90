@@ -24,6 +24,7 @@ module pcmx_iface
     real(c_double) :: cs0, gamma, cp, rho0, mu0
     real(c_double) :: cdt, cdtv
     integer(c_int) :: ldt, precision, kernel
+    integer(c_int) :: lupw_uu, lupw_lnrho
   end type pcmx_params
 
   interface

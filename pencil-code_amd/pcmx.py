@@ -48,6 +48,7 @@ class Params(ctypes.Structure):
         ("rho0", ctypes.c_double), ("mu0", ctypes.c_double),
         ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
         ("ldt", ctypes.c_int), ("precision", ctypes.c_int), ("kernel", ctypes.c_int),
+        ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
     ]
 
 
@@ -60,12 +61,23 @@ class Box(ctypes.Structure):
     ]
 
 
+class Forcing(ctypes.Structure):
+    """Helical forcing parameters (pcmx_forcing)."""
+    _fields_ = [
+        ("active", ctypes.c_int), ("target", ctypes.c_int), ("scale_sqrt_dt", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("kk", ctypes.c_double * 3), ("phase", ctypes.c_double), ("fact", ctypes.c_double),
+        ("coef1", ctypes.c_double * 3), ("coef2", ctypes.c_double * 3), ("fda", ctypes.c_double * 3),
+        ("k1_ff", ctypes.c_double), ("xyz0", ctypes.c_double * 3),
+    ]
+
+
 ENS_NO_GRAPH = 1
 
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
                 cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
-                kernel=None, L=2 * math.pi):
+                kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False):
     """Parameters of one periodic box with the reference defaults
     (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
     registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
@@ -80,7 +92,7 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, c
     iax = (5 if lentropy else 4) if lmagnetic else -1
     mvar = 4 + (1 if lentropy else 0) + (3 if lmagnetic else 0)
     return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, Lz / nz, nu, eta, chi, cs0, gamma,
-                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel)
+                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho))
 
 
 _lib = None
@@ -125,6 +137,7 @@ def lib():
             "pcmx_ens_set_timing": [ctypes.c_int],
             "pcmx_ens_timing": [dp],
             "pcmx_write_snapshot": [ctypes.c_char_p, ctypes.c_double, dp],
+            "pcmx_set_forcing": [ctypes.POINTER(Forcing)],
             "pcmx_write_record": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long),
                                   ctypes.c_int, ctypes.c_long],
         }
@@ -227,6 +240,23 @@ def time_step(nsteps, t, dt, ldiag=True):
     return tt.value, dd.value
 
 
+def set_forcing(kk, phase, fact, coef1, coef2, fda=(1.0, 1.0, 1.0), k1_ff=1.0,
+                xyz0=(-math.pi, -math.pi, -math.pi), target=0, scale_sqrt_dt=False):
+    """Helical forcing added after every following step (pcmx_set_forcing):
+    the parameters forcing_pars_hel returns (forcing.f90:1731-1746), k1_ff,
+    the box origin and the 0-based slot of the forced vector (iux, or iax
+    for lmagnetic_forcing).  With scale_sqrt_dt, fact is force/ffnorm and
+    each step multiplies it by sqrt(dt)."""
+    v3 = lambda a: (ctypes.c_double * 3)(*[float(x) for x in a])  # noqa: E731
+    fr = Forcing(1, int(target), int(bool(scale_sqrt_dt)), 0, v3(kk), float(phase), float(fact), v3(coef1),
+                 v3(coef2), v3(fda), float(k1_ff), v3(xyz0))
+    _check(lib().pcmx_set_forcing(ctypes.byref(fr)))
+
+
+def clear_forcing():
+    _check(lib().pcmx_set_forcing(ctypes.byref(Forcing())))
+
+
 def write_snapshot(path, t, xyz0=(-math.pi, -math.pi, -math.pi)):
     """Writes the device state as a var.dat snapshot (the reference's
     output_snap layout for one process, src/io_dist.f90:108-199) without
@@ -282,12 +312,15 @@ def interior_of(f, mvar):
     return f[:mvar, g:-g, g:-g, g:-g]
 
 
-def advance(interior, par, nsteps, dt, t=0.0, per_substep=False):
+def advance(interior, par, nsteps, dt, t=0.0, per_substep=False, forcing=None):
     """Convenience driver in the reference's call order: load, nsteps steps,
-    copy back.  Returns (interior_after, t, dt)."""
+    copy back.  Returns (interior_after, t, dt).  forcing: keyword
+    arguments of set_forcing, applied after every step."""
     f = farray_from_interior(interior)
     initialize_gpu(f, par, t=t, nt=1)
     try:
+        if forcing:
+            set_forcing(**forcing)
         if per_substep:
             for _ in range(nsteps):
                 ds = 0.0

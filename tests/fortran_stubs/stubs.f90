@@ -59,6 +59,20 @@ contains
     real :: dt1_
   end subroutine
 end module Sub
+module Hydro
+  implicit none
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(200) :: p_par
+  end subroutine
+end module Hydro
+module Density
+  implicit none
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(100) :: p_par
+  end subroutine
+end module Density
 module File_io
   implicit none
   integer :: parallel_unit=10

@@ -36,7 +36,8 @@ def oracle_params(gp, dt):
     return O.make_params((gp.nx, gp.ny, gp.nz), lmagnetic=mag, lentropy=ent, nu=gp.nu, eta=gp.eta,
                          chi=gp.chi, cs0=gp.cs0, gamma=gp.gamma, cp=gp.cp, rho0=gp.rho0, mu0=gp.mu0,
                          cdt=gp.cdt, cdtv=gp.cdtv, dt=0.0 if gp.ldt else dt,
-                         L=(gp.dx * gp.nx, gp.dy * gp.ny, gp.dz * gp.nz))
+                         L=(gp.dx * gp.nx, gp.dy * gp.ny, gp.dz * gp.nz),
+                         lupw_uu=bool(gp.lupw_uu), lupw_lnrho=bool(gp.lupw_lnrho))
 
 
 def announce(msg):
@@ -63,11 +64,12 @@ CASES = {
 }
 
 
-def setup_case(name, precision=pcmx.PREC_FP64, kernel=None, seed=7):
+def setup_case(name, precision=pcmx.PREC_FP64, kernel=None, seed=7, lupw_uu=False, lupw_lnrho=False):
     c = CASES[name]
     gp = pcmx.make_params(c["n"], lmagnetic=c["mag"], lentropy=c["ent"], nu=c["nu"], eta=c["eta"],
                           chi=c["chi"], gamma=c["gamma"], ldt=c["ldt"], cdt=c.get("cdt", 0.9),
-                          precision=precision, kernel=kernel, L=c.get("L", 2 * math.pi))
+                          precision=precision, kernel=kernel, L=c.get("L", 2 * math.pi),
+                          lupw_uu=lupw_uu, lupw_lnrho=lupw_lnrho)
     f0 = pcmx_ics.config_state(c["kind"], c["n"], seed, lmagnetic=c["mag"], lentropy=c["ent"])
     return c, gp, f0
 
