@@ -142,8 +142,11 @@ int pcmx_split_update(void);
 int pcmx_time_step(int nsteps, double* t, double* dt, int ldiag);
 
 /* Times the tile shapes and z-chunk lengths the grid allows for the split
- * kernel (kernel 3) on the current state, without changing it, and keeps the
- * fastest.  best[0] = ms per launch of the chosen plan, best[1] = tile width,
+ * kernel (kernel 3) on the current state, without changing it (the three
+ * stages of one step, as pcmx_time_step runs them for fixed or Courant dt,
+ * each reading the state and writing only the spare buffer and the 2N
+ * registers), and keeps the fastest.  Call it between steps.
+ * best[0] = mean ms per launch of the chosen plan, best[1] = tile width,
  * best[2] = z-chunk length (best may be NULL).  No counterpart in the
  * reference (its GPU library fixes its launch shapes at build time). */
 int pcmx_tune(double* best);

@@ -1,7 +1,8 @@
 // pcmx_aux.h -- the small kernels around the RHS kernels: the reduction
-// finaliser, the explicit first-substep update for Courant-limited steps, and
-// the layout conversion between the reference f-array (with ghost zones) and
-// device state.  Included by the host translation unit only.
+// finaliser, the explicit first-substep update for Courant-limited steps, the
+// layout conversion between the reference f-array (with ghost zones) and
+// device state, and the separate helical-forcing pass.  Included by the host
+// translation unit only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -88,34 +89,22 @@ __global__ void __launch_bounds__(256) k_unpack_interior(const double* __restric
   }
 }
 
-// Helical forcing of one step (forcing_hel, forcing.f90:2066-2474, default
-// path: force_ampl = 1, cos(omega_ff t) = 1, force1_scl = 1).  tab holds the
-// unit phasors exp(i(kk1 k1_ff x_l + phase)) (nx of them), exp(i kk2 k1_ff y_m)
-// (ny) and exp(i kk3 k1_ff z_n) (nz) as (re, im); fx = fact * phasor_x
-// (fxyz_coefs_hel, :2003-2064).  fxyz = (fx fy) fz, then
-// f(:,target+j) += fda_j Re((coef1_j + i coef2_j) fxyz).
-struct ForceArgs {
-  double c1[3], c2[3], fda[3];
-  double fact;
-  int target;
-};
-
+// Helical forcing of one step as a separate pass over the forced vector
+// (kernels without the fused epilogue); the term is force_term's.
 __global__ void __launch_bounds__(256) k_add_forcing(double* __restrict__ f, const double2* __restrict__ tab,
-                                                     ForceArgs fa, int nx, int ny, int nz) {
+                                                     double fact, int target, int nx, int ny, int nz) {
   const int l = blockIdx.x * 64 + (threadIdx.x & 63);
   const int m = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int n = blockIdx.z;
   if (l >= nx || m >= ny) return;
   const long nxyz = (long)nx * ny * nz;
-  double2 fx = tab[l];
-  fx.x *= fa.fact;
-  fx.y *= fa.fact;
-  const double2 fy = tab[nx + m], fz = tab[nx + ny + n];
-  const double ar = fx.x * fy.x - fx.y * fy.y, ai = fx.x * fy.y + fx.y * fy.x;
-  const double br = ar * fz.x - ai * fz.y, bi = ar * fz.y + ai * fz.x;
-  double* p = f + fa.target * nxyz + ((long)n * ny + m) * nx + l;
+  double* p = f + target * nxyz + ((long)n * ny + m) * nx + l;
+  double v[3];
 #pragma unroll
-  for (int j = 0; j < 3; ++j) p[j * nxyz] += fa.fda[j] * (fa.c1[j] * br - fa.c2[j] * bi);
+  for (int j = 0; j < 3; ++j) v[j] = p[j * nxyz];
+  force_term(tab, fact, nx, ny, nz, l, m, n, v);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) p[j * nxyz] = v[j];
 }
 
 }  // namespace pcmx

@@ -200,6 +200,22 @@ int finalize_red(Engine& e, bool set_dt, bool set_means) {
   return PCMX_OK;
 }
 
+// fact of the forcing of a step with time step dt: force/ffnorm*sqrt(dt)
+// when the library applies the dt factor (forcing.f90:1958).
+double forcing_fact(const Engine& e, double dt) {
+  return e.frc.scale_sqrt_dt ? e.frc.fact * std::sqrt(dt) : e.frc.fact;
+}
+
+// The split kernel adds the forcing in the epilogue of the last substep
+// (without entropy; with it, a separate pass follows the step).
+bool forcing_fused(const Engine& e) { return e.kernel == 3 && !e.ent; }
+void fuse_forcing(const Engine& e, KArgs& a, double dt) {
+  if (!forcing_fused(e) || !e.frc.active) return;
+  a.ftab = e.ftab;
+  a.ffact = forcing_fact(e, dt);
+  a.ftarget = e.frc.target;
+}
+
 // One RK substep s (0-based) on the device; no host synchronisation.
 // dt_host is the fixed step (ignored when ldt).  diag requests the
 // reductions of the first substep.
@@ -233,6 +249,7 @@ int substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullpt
     a.mode = MODE_UPDATE;
     a.d_dt = ldt ? e.results + R_DT : nullptr;
     a.partials = (s == 0 && diag) ? e.partials : nullptr;
+    if (s == 2) fuse_forcing(e, a, dt_host);
     if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
     int rc = launch_rhs(e, a, false);
     if (rc) return rc;
@@ -275,6 +292,7 @@ int substep_deferred(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0
     a.stage = ST_S3;
     a.mode = MODE_UPDATE;
     a.w = e.w2;
+    fuse_forcing(e, a, dt_host);
   }
   if (ev0) HIPCHECK(hipEventRecord(ev0, e.stream));
   int rc = launch_rhs(e, a, s == 0 && ldt);
@@ -294,18 +312,9 @@ int substep_deferred(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0
 // The forcing of the step just completed, added to the state (addforce after
 // time_step, run.f90:575).  dt is the step's dt, known on the host by now.
 int add_forcing(Engine& e, double dt) {
-  const pcmx_forcing& fr = e.frc;
-  ForceArgs fa{};
-  for (int j = 0; j < 3; ++j) {
-    fa.c1[j] = fr.coef1[j];
-    fa.c2[j] = fr.coef2[j];
-    fa.fda[j] = fr.fda[j];
-  }
-  fa.fact = fr.scale_sqrt_dt ? fr.fact * std::sqrt(dt) : fr.fact;  // force/ffnorm*sqrt(dt), forcing.f90:1958
-  fa.target = fr.target;
   const dim3 grid((e.par.nx + 63) / 64, (e.par.ny + 3) / 4, e.par.nz);
-  hipLaunchKernelGGL(k_add_forcing, grid, dim3(256), 0, e.stream, e.f[e.cur], e.ftab, fa, e.par.nx, e.par.ny,
-                     e.par.nz);
+  hipLaunchKernelGGL(k_add_forcing, grid, dim3(256), 0, e.stream, e.f[e.cur], e.ftab, forcing_fact(e, dt),
+                     e.frc.target, e.par.nx, e.par.ny, e.par.nz);
   HIPCHECK(hipGetLastError());
   return PCMX_OK;
 }
@@ -313,7 +322,7 @@ int add_forcing(Engine& e, double dt) {
 int run_substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   const int rc = (e.kernel == 3 && e.par.ldt) ? substep_deferred(e, s, dt_host, diag, ev0, ev1)
                                               : substep(e, s, dt_host, diag, ev0, ev1);
-  if (rc || s != 2 || !e.frc.active) return rc;
+  if (rc || s != 2 || !e.frc.active || forcing_fused(e)) return rc;
   return add_forcing(e, dt_host);
 }
 
@@ -709,17 +718,45 @@ int pcmx_tune(double* best_ms) {
       if (lz <= e.par.nz && e.par.nz % lz == 0) cands.push_back({tile, lz});
     if (e.par.nz > 1024 || e.par.nz % 8) cands.push_back({tile, e.par.nz});
   }
-  // time the last-substep stage: it reads f[cur] and w and writes only the
-  // spare buffer, so the state is untouched
+  // Time the three stages a step runs (fixed dt: S1, S2, S3; Courant steps:
+  // W, S2D, S3), each reading the state f[cur] and writing only the spare
+  // buffer and the 2N registers, which hold nothing between steps, so the
+  // state is untouched.
   if (e.pending_f1) return fail(PCMX_ERR_STATE, "pcmx_tune between substeps 1 and 2 of a step");
-  KArgs a = base_args(e);
-  a.fin = e.f[e.cur];
-  a.fout = e.f[1 - e.cur];
-  a.alpha = ALPHA_TS[2];
-  a.beta = BETA_TS[2];
-  a.dtb = 0.0;
-  a.stage = ST_S3;
-  a.mode = MODE_UPDATE;
+  const bool ldt = e.par.ldt != 0;
+  KArgs st[3];
+  for (int s = 0; s < 3; ++s) {
+    KArgs& a = st[s];
+    a = base_args(e);
+    a.fin = e.f[e.cur];
+    a.fout = e.f[1 - e.cur];
+    a.alpha = ALPHA_TS[s];
+    a.beta = BETA_TS[s];
+    a.dtb = 0.0;
+    a.beta1 = BETA_TS[0];
+    a.dtb1 = 0.0;
+    a.first = (s == 0);
+    a.write_w = (s < 2);
+    a.mode = MODE_UPDATE;
+    a.stage = s == 0 ? ST_S1 : (s == 1 ? ST_S2 : ST_S3);
+    a.d_dt = ldt ? e.results + R_DT : nullptr;
+  }
+  if (ldt) {
+    st[0].stage = ST_W;
+    st[0].mode = MODE_W;
+    st[0].fout = nullptr;
+    st[1].stage = ST_S2D;
+    st[1].wout = e.w2;
+    st[2].w = e.w2;
+  }
+  auto run_step = [&]() -> int {
+    for (int s = 0; s < 3; ++s) {
+      st[s].partials = (s == 0) ? e.partials : nullptr;
+      const int r = launch_rhs(e, st[s], ldt && s == 0);
+      if (r) return r;
+    }
+    return PCMX_OK;
+  };
   const int tile0 = e.tile;
   const ZPlan zp0 = e.zplan;
   hipEvent_t t0, t1;
@@ -732,9 +769,9 @@ int pcmx_tune(double* best_ms) {
     e.tile = c.first;
     e.zplan = tile_plan(e.par.nx, e.par.ny, e.par.nz, tile_tx(c.first), tile_ty(c.first), c.second);
     if (e.zplan.nblocks > e.partials_cap) continue;
-    if ((rc = launch_rhs(e, a, false))) break;  // warm-up
+    if ((rc = run_step())) break;  // warm-up
     HIPCHECK(hipEventRecord(t0, e.stream));
-    for (int it = 0; it < 2 && !rc; ++it) rc = launch_rhs(e, a, false);
+    for (int it = 0; it < 2 && !rc; ++it) rc = run_step();
     if (rc) break;
     HIPCHECK(hipEventRecord(t1, e.stream));
     HIPCHECK(hipEventSynchronize(t1));
@@ -752,7 +789,7 @@ int pcmx_tune(double* best_ms) {
   e.zplan = rc ? zp0 : bplan;
   if (rc) return rc;
   if (best_ms) {
-    best_ms[0] = best / 2.0;
+    best_ms[0] = best / 6.0;  // per launch
     best_ms[1] = tile_tx(btile);
     best_ms[2] = bplan.lz;
   }
@@ -850,7 +887,7 @@ int pcmx_set_forcing(const pcmx_forcing* frc) {
     return fail(PCMX_ERR_ARG, "pcmx_forcing: non-finite parameter");
   const int n[3] = {p.nx, p.ny, p.nz};
   const double d[3] = {p.dx, p.dy, p.dz};
-  std::vector<double2> tab((size_t)p.nx + p.ny + p.nz);
+  std::vector<double2> tab((size_t)p.nx + p.ny + p.nz + FORCE_NCOEF / 2);
   size_t o = 0;
   for (int a = 0; a < 3; ++a)
     for (int l = 0; l < n[a]; ++l) {
@@ -863,6 +900,13 @@ int pcmx_set_forcing(const pcmx_forcing* frc) {
       const double th = frc->kk[a] * frc->k1_ff * c + (a == 0 ? frc->phase : 0.0);
       tab[o++] = make_double2(std::cos(th), std::sin(th));
     }
+  double* cf = reinterpret_cast<double*>(tab.data() + o);
+  for (int j = 0; j < 3; ++j) {
+    cf[j] = frc->coef1[j];
+    cf[3 + j] = frc->coef2[j];
+    cf[6 + j] = frc->fda[j];
+  }
+  cf[9] = 0.0;
   if (!e.ftab) HIPCHECK(hipMalloc(&e.ftab, tab.size() * sizeof(double2)));
   HIPCHECK(hipMemcpyAsync(e.ftab, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice, e.stream));
   HIPCHECK(hipStreamSynchronize(e.stream));

@@ -40,7 +40,34 @@ struct KArgs {
   int stage;
   double beta1;         // beta_ts(1)
   double dtb1;          // dt*beta_ts(1) when d_dt == nullptr
+  // helical forcing added to the last substep's output (split kernel): the
+  // force_term table or nullptr, fact of this step, first forced slot
+  const double2* ftab;
+  double ffact;
+  int ftarget;
 };
+
+// Helical forcing at cell (l, m, n) (forcing_hel, forcing.f90:2066-2474,
+// default path: force_ampl = 1, cos(omega_ff t) = 1, force1_scl = 1).  tab
+// holds the unit phasors exp(i(kk1 k1_ff x_l + phase)) (nx of them),
+// exp(i kk2 k1_ff y_m) (ny) and exp(i kk3 k1_ff z_n) (nz) as (re, im),
+// followed by coef1[3], coef2[3], fda[3] (FORCE_NCOEF doubles);
+// fx = fact * phasor_x (fxyz_coefs_hel, :2003-2064), fxyz = (fx fy) fz, and
+// component j of the force is fda_j Re((coef1_j + i coef2_j) fxyz).  f[j] is
+// the state after the step; returns f[j] + force_j.  Written with explicit
+// fused multiply-adds so that the separate forcing pass and the split
+// kernel's fused epilogue round identically.
+constexpr int FORCE_NCOEF = 10;  // 9 used, padded to whole double2 entries
+__device__ __forceinline__ void force_term(const double2* tab, double fact, int nx, int ny, int nz, int l, int m,
+                                           int n, double (&f)[3]) {
+  const double2 px = tab[l], fy = tab[nx + m], fz = tab[nx + ny + n];
+  const double* cf = reinterpret_cast<const double*>(tab + nx + ny + nz);
+  const double fxr = px.x * fact, fxi = px.y * fact;
+  const double ar = fma(fxr, fy.x, -(fxi * fy.y)), ai = fma(fxr, fy.y, fxi * fy.x);
+  const double br = fma(ar, fz.x, -(ai * fz.y)), bi = fma(ar, fz.y, ai * fz.x);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) f[j] = fma(cf[6 + j], fma(cf[j], br, -(cf[3 + j] * bi)), f[j]);
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

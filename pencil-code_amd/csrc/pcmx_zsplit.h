@@ -725,8 +725,14 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
         if constexpr (ENT) xo[XJ2 * TL::NT + t] = fo.j2;
         if constexpr (DT) xo[XVA * TL::NT + t] = fo.advec_va2;
         if constexpr (G::UPDATE) {
+          double fo3[3];
 #pragma unroll
-          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(fa[i] + dtb * wa[i], &vplane(a.fout, IA + i, q)[own]);
+          for (int i = 0; i < 3; ++i) fo3[i] = fa[i] + dtb * wa[i];
+          if constexpr (ST == ST_S3 && !ENT)
+            if (a.ftab && a.ftarget == IA)
+              force_term(a.ftab, a.ffact, nx, a.ny, a.nz, x, y, q, fo3);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) __builtin_nontemporal_store(fo3[i], &vplane(a.fout, IA + i, q)[own]);
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
@@ -755,8 +761,19 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
         double dt1 = 0.0;
         flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
         if constexpr (G::UPDATE) {
+          double fo3[NF];
 #pragma unroll
-          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(ff[i] + dtb * wf[i], &vplane(a.fout, var_of(0, i), q)[own]);
+          for (int i = 0; i < NF; ++i) fo3[i] = ff[i] + dtb * wf[i];
+          if constexpr (ST == ST_S3 && !ENT) {  // with entropy: a separate pass (registers)
+            if (a.ftab && a.ftarget == 0) {
+              double fu[3] = {fo3[0], fo3[1], fo3[2]};
+              force_term(a.ftab, a.ffact, nx, a.ny, a.nz, x, y, q, fu);
+#pragma unroll
+              for (int i = 0; i < 3; ++i) fo3[i] = fu[i];
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < NF; ++i) __builtin_nontemporal_store(fo3[i], &vplane(a.fout, var_of(0, i), q)[own]);
         }
         if constexpr (G::WRITEW) {
 #pragma unroll
