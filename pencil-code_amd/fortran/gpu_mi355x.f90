@@ -33,8 +33,10 @@ module GPU
   integer :: it_test_rhs=1
   integer :: gpu_precision=0      !PAR-DOC: 0 FP64, 1 mixed (FP32 stencils on box-mean offsets)
   integer :: gpu_kernel=0         !PAR-DOC: 0 auto, 1 direct, 2 z-marching LDS kernel, 3 split z-marching kernel
+  logical :: lgpu_magnetic_forcing=.false. !PAR-DOC: the helical force acts on aa (set with forcing_run_pars lmagnetic_forcing=T)
 !
-  namelist /gpu_run_pars/ ltest_bcs, ltest_rhs, it_test_rhs, gpu_precision, gpu_kernel
+  namelist /gpu_run_pars/ ltest_bcs, ltest_rhs, it_test_rhs, gpu_precision, gpu_kernel, &
+                          lgpu_magnetic_forcing
 !
   real, dimension(n_gpu_reduced_vars) :: reduced_last = 0.
 !
@@ -187,6 +189,7 @@ contains
       real(c_double), dimension(PCMX_N_REDUCED) :: red
 !
       call keep_compiler_quiet(f)
+      if (lforcing .and. isubstep == itorder) call update_forcing_gpu
       dt_c = dt
       call pcmx_check(pcmx_rhs(isubstep, real(t, c_double), dt_c), 'rhs_GPU')
       if (isubstep == 1) then
@@ -196,6 +199,57 @@ contains
       endif
 !
     endsubroutine rhs_GPU
+!***********************************************************************
+    subroutine update_forcing_gpu
+!
+!  A new helical force for the step in progress, drawn by the reference's
+!  own forcing_pars_hel (forcing.f90:1731-1746, with this step's dt) and
+!  handed to the device, which adds it after the last substep; the
+!  reference's GPU path does the same at its last substep
+!  (astaroth/gpu_astaroth.cc:1589-1602, astaroth/forcing.h:13-29).  Only
+!  iforce='helical' on its default path is provided.
+!
+      use Forcing, only: forcing_pars_hel, forcing_pushpars2c => pushpars2c
+!
+      integer(KIND=ikind8), dimension(200) :: p_par
+      logical, dimension(:), pointer :: lhel
+      logical, pointer :: lsecond
+      real, pointer :: k1_ff_p
+      integer, pointer :: zsym
+      real, dimension(3) :: coef1, coef2, coef3, fda, kk
+      real :: phase, fact
+      type(pcmx_forcing) :: frc
+!
+!  Flags and parameters through the module's pushpars2c list
+!  (forcing.f90:6599-6712): k1_ff (1), iforcing_zsym (3), lforce_helical (82),
+!  lsecond_force (83).
+!
+      p_par = 0
+      call forcing_pushpars2c(p_par)
+      call c_f_pointer(transfer(p_par(82), c_null_ptr), lhel, [2])
+      call c_f_pointer(transfer(p_par(83), c_null_ptr), lsecond)
+      call c_f_pointer(transfer(p_par(1), c_null_ptr), k1_ff_p)
+      call c_f_pointer(transfer(p_par(3), c_null_ptr), zsym)
+      if (.not. lhel(1)) call fatal_error('update_forcing_gpu', 'only iforce=''helical'' runs on the GPU')
+      if (lsecond) call fatal_error('update_forcing_gpu', 'a second force is not provided on the GPU')
+      if (zsym /= 0) call fatal_error('update_forcing_gpu', 'iforcing_zsym /= 0 is not provided on the GPU')
+!
+      call forcing_pars_hel(coef1, coef2, coef3, fda, kk, phase, fact)
+      frc%active = 1
+      frc%target = merge(iax - 1, iux - 1, lgpu_magnetic_forcing)
+      frc%scale_sqrt_dt = 0
+      frc%reserved = 0
+      frc%kk = kk
+      frc%phase = phase
+      frc%fact = fact
+      frc%coef1 = coef1
+      frc%coef2 = coef2
+      frc%fda = fda
+      frc%k1_ff = k1_ff_p
+      frc%xyz0 = xyz0
+      call pcmx_check(pcmx_set_forcing(frc), 'update_forcing_gpu')
+!
+    endsubroutine update_forcing_gpu
 !***********************************************************************
     subroutine before_boundary_gpu(f, lrmv, isubstep, t)
 !

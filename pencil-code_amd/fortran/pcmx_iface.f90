@@ -27,6 +27,15 @@ module pcmx_iface
     integer(c_int) :: lupw_uu, lupw_lnrho
   end type pcmx_params
 
+  ! pcmx_forcing of include/pcmx.h: helical forcing after each step
+  type, bind(C) :: pcmx_forcing
+    integer(c_int) :: active, target, scale_sqrt_dt, reserved
+    real(c_double) :: kk(3), phase, fact
+    real(c_double) :: coef1(3), coef2(3), fda(3)
+    real(c_double) :: k1_ff
+    real(c_double) :: xyz0(3)
+  end type pcmx_forcing
+
   interface
     type(c_ptr) function pcmx_last_error() bind(C, name='pcmx_last_error')
       import :: c_ptr
@@ -107,6 +116,10 @@ module pcmx_iface
       character(kind=c_char), dimension(*), intent(in) :: path
       real(c_double), value :: t
       real(c_double), dimension(3), intent(in) :: xyz0
+    end function
+    integer(c_int) function pcmx_set_forcing(frc) bind(C, name='pcmx_set_forcing')
+      import :: c_int, pcmx_forcing
+      type(pcmx_forcing), intent(in) :: frc
     end function
   end interface
 

@@ -8,7 +8,7 @@ module Cparam
   integer, parameter :: nx=32, ny=32, nz=32, mx=nx+6, my=ny+6, mz=nz+6
   integer, parameter :: mvar=7, mfarray=7, ncpus=1
   logical, parameter :: lhydro=.true., ldensity=.true., lmagnetic=.true., lentropy=.false.
-  logical, parameter :: ltemperature=.false., lthermal_energy=.false.
+  logical, parameter :: ltemperature=.false., lthermal_energy=.false., lforcing=.true.
 end module Cparam
 module Cdata
   use Cparam
@@ -19,6 +19,7 @@ module Cdata
   logical :: loutput_varn_at_exact_tsnap=.false.
   logical, dimension(3) :: lperi=.true., lequidist=.true.
   integer :: iux=1, ilnrho=4, iss=0, iax=5, itorder=3, nt=10
+  real, dimension(3) :: xyz0=-3.14159
 end module Cdata
 module General
   implicit none
@@ -73,6 +74,17 @@ contains
     integer(kind=selected_int_kind(14)), dimension(100) :: p_par
   end subroutine
 end module Density
+module Forcing
+  implicit none
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(100) :: p_par
+  end subroutine
+  subroutine forcing_pars_hel(coef1, coef2, coef3, fda, kk, phase, fact)
+    real, dimension(3), intent(out) :: coef1, coef2, coef3, fda, kk
+    real, intent(out) :: phase, fact
+  end subroutine
+end module Forcing
 module File_io
   implicit none
   integer :: parallel_unit=10
