@@ -61,7 +61,17 @@ typedef struct pcmx_params {
                                 sub.f90:7541-7574); the split kernel takes it together with
                                 lupw_lnrho and without entropy, the direct kernel in any case */
   int lupw_lnrho;            /* density_run_pars lupw_lnrho (density.f90:2772-2773) */
+  double gravz;              /* grav_run_pars gravz, gravz_profile='const' (gravity_simple.f90:
+                                addgravity :1083-1162): df(uz) += gravz after the magnetic terms */
+  int lperi_z;               /* init_pars lperi(3): 1 periodic z; 0 the bcz rules below, with
+                                boundary points at the first and last z plane (dz = Lz/(nz-1)) */
+  int bcz_bot[8], bcz_top[8];/* run_pars bcz per variable slot when lperi_z == 0: PCMX_BC_* */
 } pcmx_params;
+
+/* z boundary conditions of bc_sym_z (boundcond.f90:3842-3900) */
+#define PCMX_BC_S 1  /* 's':  f(n1-i) = f(n1+i) */
+#define PCMX_BC_A 2  /* 'a':  f(n1-i) = -f(n1+i), f(n1) = 0 */
+#define PCMX_BC_A2 3 /* 'a2': f(n1-i) = 2 f(n1) - f(n1+i) */
 
 /* Last error message (thread-local to the calling thread). */
 const char* pcmx_last_error(void);

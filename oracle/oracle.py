@@ -32,7 +32,24 @@ class Params(ctypes.Structure):
         ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
         ("dt", ctypes.c_double),
         ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
+        ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
+        ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
     ]
+
+
+BC_CODES = {"s": 1, "a": 2, "a2": 3}
+
+
+def bc_codes(bcz, nv):
+    """Per-variable (bottom, top) codes from run.in style entries: 'a2' or
+    'a2:s' (bottom:top)."""
+    if len(bcz) != nv:
+        raise ValueError(f"{len(bcz)} z boundary conditions for {nv} variables")
+    bot, top = (ctypes.c_int * 8)(), (ctypes.c_int * 8)()
+    for k, b in enumerate(bcz):
+        lo, _, hi = b.partition(":")
+        bot[k], top[k] = BC_CODES[lo], BC_CODES[hi or lo]
+    return bot, top
 
 
 class Forcing(ctypes.Structure):
@@ -51,13 +68,17 @@ def make_forcing(kk, phase, fact, coef1, coef2, fda=(1.0, 1.0, 1.0), k1_ff=1.0, 
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
                 cs0=1.0, gamma=1.0, cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25,
-                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False):
+                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None):
     """Run parameters with the reference defaults (cdata.f90:235-239,
-    eos_idealgas.f90:40-46)."""
+    eos_idealgas.f90:40-46).  bcz: None (periodic z) or one run.in entry per
+    variable ('s', 'a', 'a2', or 'bottom:top')."""
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
+    nv = 4 + int(lentropy) + 3 * int(lmagnetic)
+    bot, top = bc_codes(bcz, nv) if bcz is not None else ((ctypes.c_int * 8)(), (ctypes.c_int * 8)())
     return Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
-                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho))
+                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho),
+                  gravz, int(bcz is None), bot, top)
 
 
 def build():
@@ -86,8 +107,9 @@ def lib():
         L.pco_pde.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, ctypes.c_int]
         L.pco_advance.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
         L.pco_addforce.argtypes = [dp, pp, ctypes.POINTER(Forcing)]
+        L.pco_boundconds.argtypes = [dp, pp]
         for name in ("pco_der", "pco_der2", "pco_derij", "pco_fill_ghosts", "pco_pde",
-                     "pco_advance", "pco_eos_constants", "pco_addforce"):
+                     "pco_advance", "pco_eos_constants", "pco_addforce", "pco_boundconds"):
             getattr(L, name).restype = None
         _lib = L
     return _lib
@@ -188,3 +210,10 @@ def addforce(interior_f, p, forcing):
     f = with_ghosts(interior_f)
     lib().pco_addforce(_ptr(f), ctypes.byref(p), ctypes.byref(forcing))
     return interior(f).copy()
+
+
+def boundconds(f, p):
+    """The run's boundary conditions applied in place to an f-array with
+    ghost zones (nvar, mz, my, mx): periodic x and y, periodic or bcz in z."""
+    lib().pco_boundconds(_ptr(f), ctypes.byref(p))
+    return f

@@ -85,8 +85,8 @@ static geom make_geom(const pco_params* p) {
   g.nx = p->nx; g.ny = p->ny; g.nz = p->nz;
   g.mx = p->nx + 2 * NG; g.my = p->ny + 2 * NG; g.mz = p->nz + 2 * NG;
   g.mw = (long)g.mx * g.my * g.mz;
-  /* grid.f90:123-125 (periodic: dx = Lx/nxgrid), dx_1 = 1/xprim (:495) */
-  double dx = p->Lx / p->nx, dy = p->Ly / p->ny, dz = p->Lz / p->nz;
+  /* grid.f90:123-125 (periodic: dx = Lx/nxgrid, else Lx/(nxgrid-1)), dx_1 = 1/xprim (:495) */
+  double dx = p->Lx / p->nx, dy = p->Ly / p->ny, dz = p->lperi_z ? p->Lz / p->nz : p->Lz / (p->nz - 1);
   g.dx_1 = 1.0 / dx; g.dy_1 = 1.0 / dy; g.dz_1 = 1.0 / dz;
   /* grid.f90:1063 (lmaximal_cdtv=F): dline_1(:,1)**2+dline_1(:,2)**2+dline_1(:,3)**2 */
   g.dxyz_2 = g.dx_1 * g.dx_1 + g.dy_1 * g.dy_1 + g.dz_1 * g.dz_1;
@@ -95,6 +95,39 @@ static geom make_geom(const pco_params* p) {
 
 #define F(g, f, ix, iy, iz, k) \
   (f)[(long)(k) * (g)->mw + ((long)(iz) * (g)->my + (iy)) * (g)->mx + (ix)]
+
+/* ------------------------------------------------------------------ */
+/* bc_sym_z (boundcond.f90:3842-3900) for one variable a (full x,y extent),
+ * bottom then top, with the codes of boundconds_z ('s' :1486-1488, 'a'
+ * :1498-1500, 'a2' :1501-1503). */
+static void bc_sym_z(double* a, int mx, int my, int nz, int code, int top) {
+  const long pl = (long)mx * my;
+  const int nb = top ? NG + nz - 1 : NG; /* n2 or n1 */
+  for (int i = 1; i <= NG; ++i) {
+    double* gh = a + (long)(top ? nb + i : nb - i) * pl;
+    const double* in = a + (long)(top ? nb - i : nb + i) * pl;
+    const double* bd = a + (long)nb * pl;
+    for (long q = 0; q < pl; ++q) {
+      if (code == PCO_BC_S) gh[q] = in[q];
+      else if (code == PCO_BC_A) gh[q] = -in[q];
+      else if (top) gh[q] = bd[q] + (bd[q] - in[q]);  /* f(n2)+(f(n2)+sgn*f(n2-i)) */
+      else gh[q] = 2 * bd[q] - in[q];                  /* 2*f(n1)+sgn*f(n1+i) */
+    }
+  }
+  if (code == PCO_BC_A) memset(a + (long)nb * pl, 0, sizeof(double) * pl); /* set bdry value=0 */
+}
+
+void pco_boundconds(double* f, const pco_params* p) {
+  const int nvar = pco_nvar(p);
+  pco_fill_ghosts(f, p->nx, p->ny, p->nz, nvar);
+  if (p->lperi_z) return;
+  const int mx = p->nx + 2 * NG, my = p->ny + 2 * NG;
+  const long mw = (long)mx * my * (p->nz + 2 * NG);
+  for (int k = 0; k < nvar; ++k) {
+    bc_sym_z(f + k * mw, mx, my, p->nz, p->bcz_bot[k], 0);
+    bc_sym_z(f + k * mw, mx, my, p->nz, p->bcz_top[k], 1);
+  }
+}
 
 /* ------------------------------------------------------------------ */
 /* periodic ghost zones: boundcond.f90:2387-2409 (x), :2411-2433 (y),
@@ -624,6 +657,12 @@ static void calc_rhs(const ctx* c, double* df, int m, int n, pencils* pc) {
       }
     }
   }
+  /* addgravity (gravity_simple.f90:1083-1162, called after daa_dt at equ.f90:1316):
+   * df(iuz) += gg(:,3), gg(:,3) = gravz for gravz_profile='const' */
+  if (p->gravz != 0.0) {
+    double* d = DF(2);
+    for (int l = 0; l < nx; ++l) d[l] = d[l] + p->gravz;
+  }
 #undef DF
 }
 
@@ -663,7 +702,7 @@ void pco_pde(double* f, double* df, const pco_params* p, int want_dt, double* dt
              int nthreads) {
   ctx c = make_ctx(p, want_dt);
   const geom* g = &c.g;
-  pco_fill_ghosts(f, g->nx, g->ny, g->nz, c.nvar);
+  pco_boundconds(f, p);
   double dt1_all = 0.0, su2 = 0.0, mu2 = 0.0, sb2 = 0.0;
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -801,7 +840,9 @@ void pco_addforce(double* f, const pco_params* p, const pco_forcing* fo) {
   const geom gg = make_geom(p);
   const geom* g = &gg;
   const int nx = g->nx, ny = g->ny, nz = g->nz;
-  const double dx = p->Lx / p->nx, dy = p->Ly / p->ny, dz = p->Lz / p->nz;
+  const double dx = p->Lx / p->nx, dy = p->Ly / p->ny;
+  const double dz = p->lperi_z ? p->Lz / p->nz : p->Lz / (p->nz - 1);
+  const double zsh = p->lperi_z ? 0.5 : 0.0; /* cell centres, or grid points from the boundary */
   double* fxr = malloc(sizeof(double) * 2 * (nx + ny + nz));
   double *fxi = fxr + nx, *fyr = fxi + nx, *fyi = fyr + ny, *fzr = fyi + ny, *fzi = fzr + nz;
   /* fxyz_coefs_hel: fx=exp(cmplx(0.,kk(1)*k1_ff*x+phase))*fact, fy, fz */
@@ -818,7 +859,7 @@ void pco_addforce(double* f, const pco_params* p, const pco_forcing* fo) {
     fyi[m] = sin(th);
   }
   for (int n = 0; n < nz; ++n) {
-    const double z = fo->z0 + (n + 0.5) * dz;
+    const double z = fo->z0 + (n + zsh) * dz;
     const double th = fo->kk[2] * fo->k1_ff * z;
     fzr[n] = cos(th);
     fzi[n] = sin(th);

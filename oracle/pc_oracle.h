@@ -47,7 +47,14 @@ typedef struct {
   double dt;               /* fixed dt; 0 => Courant-limited (timestep.f90:67) */
   int lupw_uu;             /* hydro_run_pars lupw_uu: upwinded u.grad u (hydro.f90:3662) */
   int lupw_lnrho;          /* density_run_pars lupw_lnrho: upwinded u.grad lnrho (density.f90:2772-2773) */
+  double gravz;            /* grav_run_pars gravz with gravz_profile='const' (gravity_simple.f90) */
+  int lperi_z;             /* init_pars lperi(3); 0: the bcz rules below hold in z */
+  int bcz_bot[8], bcz_top[8]; /* per variable: PCO_BC_S, PCO_BC_A, PCO_BC_A2 (run_pars bcz) */
 } pco_params;
+
+#define PCO_BC_S 1   /* 's':  symmetric about the boundary point */
+#define PCO_BC_A 2   /* 'a':  antisymmetric, boundary value 0 */
+#define PCO_BC_A2 3  /* 'a2': antisymmetric about the boundary value */
 
 /* Helical forcing of one step, as forcing_pars_hel returns it
  * (forcing.f90:1707-2001) plus the grid origin and the forced variable. */
@@ -79,6 +86,12 @@ void pco_eos_constants(const pco_params* p, double out[8]);
 /* Periodic ghost fill of every variable, boundconds_x then _y then _z over
  * full extents (boundcond.f90:2387-2433, 2509-). */
 void pco_fill_ghosts(double* f, int nx, int ny, int nz, int nvar);
+
+/* boundconds_x/y/z (equ.f90:209-224) for the run's boundary conditions:
+ * periodic in x and y, periodic or bcz in z (bc_sym_z, boundcond.f90:3842-3900;
+ * 'a' sets the boundary value to 0).  Modifies f's ghost zones and, for 'a',
+ * its boundary planes. */
+void pco_boundconds(double* f, const pco_params* p);
 
 /* Single stencil applications on a ghost-filled f for all interior points.
  * out has nx*ny*nz entries, x fastest.  k is the 0-based variable. */

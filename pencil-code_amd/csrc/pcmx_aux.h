@@ -62,17 +62,55 @@ __global__ void __launch_bounds__(256) k_axpy_dt(const double* __restrict__ fin,
 }
 
 // Device interior -> one variable of the reference f-array with periodic
-// ghost zones (bc_per_x/y/z, boundcond.f90:2387-2433,2509-).
+// ghost zones (bc_per_x/y/z, boundcond.f90:2387-2433,2509-); with a
+// non-periodic z, the z ghost planes come from gh (this variable's six
+// ghost planes, k_zbc's layout).
 __global__ void __launch_bounds__(256) k_pack_ghosts(const double* __restrict__ src, double* __restrict__ dst,
-                                                     int nx, int ny, int nz) {
+                                                     int nx, int ny, int nz, const double* __restrict__ gh) {
   const int mx = nx + 6, my = ny + 6, mz = nz + 6;
   const long mw = (long)mx * my * mz;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < mw; i += (long)gridDim.x * 256L) {
     const int ix = (int)(i % mx);
     const long r = i / mx;
     const int iy = (int)(r % my), iz = (int)(r / my);
-    const int x = (ix - 3 + nx) % nx, y = (iy - 3 + ny) % ny, z = (iz - 3 + nz) % nz;
-    dst[i] = src[((long)z * ny + y) * nx + x];
+    const int x = (ix - 3 + nx) % nx, y = (iy - 3 + ny) % ny;
+    const int zi = iz - 3;
+    if (gh && (zi < 0 || zi >= nz))
+      dst[i] = gh[((long)(zi < 0 ? zi + 3 : zi - nz + 3) * ny + y) * nx + x];
+    else
+      dst[i] = src[((long)((zi + nz) % nz) * ny + y) * nx + x];
+  }
+}
+
+// z boundary conditions of a non-periodic z (bc_sym_z, boundcond.f90:
+// 3842-3900, codes of boundconds_z :1486-1503) on the device state f
+// (interior, [v][z][y][x]): ghost planes -3..-1 and nz..nz+2 of every
+// variable into gh ([v][6][ny][nx]) and, when zero_a, f = 0 on the boundary
+// planes of 'a' variables, as boundconds_z does to the f-array before the
+// RHS.  Bottom: 's' f(-i) = f(i), 'a' -f(i), 'a2' 2*f(0) - f(i); top (n2 =
+// nz-1): 's' f(n2-i), 'a' -f(n2-i), 'a2' f(n2) + (f(n2) - f(n2-i)).
+struct ZBC {
+  int bot[8], top[8];
+};
+__global__ void __launch_bounds__(256) k_zbc(double* __restrict__ f, double* __restrict__ gh, ZBC bc, int nvar,
+                                             long nxy, int nz, int zero_a) {
+  const long q = blockIdx.x * 256L + threadIdx.x;
+  const int v = blockIdx.y;
+  if (q >= nxy || v >= nvar) return;
+  double* fv = f + (long)v * nxy * nz;
+  double* gv = gh + (long)v * 6 * nxy;
+  const int n2 = nz - 1;
+  const double fb = fv[q], ft = fv[(long)n2 * nxy + q];
+#pragma unroll
+  for (int i = 1; i <= 3; ++i) {
+    const double ib = fv[(long)i * nxy + q], it = fv[(long)(n2 - i) * nxy + q];
+    const int cb = bc.bot[v], ct = bc.top[v];
+    gv[(long)(3 - i) * nxy + q] = cb == PCMX_BC_S ? ib : (cb == PCMX_BC_A ? -ib : 2 * fb - ib);
+    gv[(long)(2 + i) * nxy + q] = ct == PCMX_BC_S ? it : (ct == PCMX_BC_A ? -it : ft + (ft - it));
+  }
+  if (zero_a) {
+    if (bc.bot[v] == PCMX_BC_A) fv[q] = 0.0;
+    if (bc.top[v] == PCMX_BC_A) fv[(long)n2 * nxy + q] = 0.0;
   }
 }
 

@@ -35,12 +35,15 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
   if (active) {
     const long nxyz = a.nxyz;
     int xo[7];
-    long yo[7], zo[7];
+    long yo[7], zo[7], zvs[7];
+    const double* zb[7];  // plane z+k-3 of variable 0, and the variable stride there
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       xo[k] = (x + k - 3 + a.nx) % a.nx;
       yo[k] = (long)((y + k - 3 + a.ny) % a.ny) * a.nx;
       zo[k] = (long)((z + k - 3 + a.nz) % a.nz) * a.nx * a.ny;
+      zb[k] = zsrc(a, 0, z + k - 3);
+      zvs[k] = zsrc(a, 1, z + k - 3) - zb[k];
     }
     double mu[NV];
 #pragma unroll
@@ -49,7 +52,7 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
     // stencil operand: value at offset (i,j,k) in -3..3 of variable v, as an
     // offset from the box mean in the mixed form
     auto st = [&](int v, int i, int j, int k) -> T {
-      const double val = fin[v * nxyz + zo[k + 3] + yo[j + 3] + xo[i + 3]];
+      const double val = zb[k + 3][v * zvs[k + 3] + yo[j + 3] + xo[i + 3]];
       if constexpr (sizeof(T) == 4) return (T)(val - mu[v]);
       else return val;
     };

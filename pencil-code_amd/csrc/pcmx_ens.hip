@@ -313,6 +313,8 @@ extern "C" {
 int pcmx_ens_create(const pcmx_params* par, const pcmx_box* boxes, int nbox, int flags) {
   if (!par || !boxes || nbox < 1) return fail(PCMX_ERR_ARG, "pcmx_ens_create needs params and at least one box");
   if (par->lupw_uu || par->lupw_lnrho) return fail(PCMX_ERR_ARG, "the ensemble does not provide upwinding");
+  if (!par->lperi_z || par->gravz != 0.0)
+    return fail(PCMX_ERR_ARG, "the ensemble runs periodic boxes without gravity");
   if (ge) {
     free_ens(ge);
     ge = nullptr;

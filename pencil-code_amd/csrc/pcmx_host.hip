@@ -64,6 +64,16 @@ int check_params(const pcmx_params* p) {
     return fail(PCMX_ERR_ARG, "unknown precision %d", p->precision);
   if (ent && p->gamma == 1.0) return fail(PCMX_ERR_ARG, "entropy needs gamma /= 1");
   if (p->cdt <= 0 || p->cdtv <= 0) return fail(PCMX_ERR_ARG, "cdt and cdtv must be positive");
+  if (p->lperi_z != 0 && p->lperi_z != 1) return fail(PCMX_ERR_ARG, "lperi_z is 0 or 1");
+  if (!std::isfinite(p->gravz)) return fail(PCMX_ERR_ARG, "gravz is not finite");
+  if (!p->lperi_z) {
+    if (p->nz < 8) return fail(PCMX_ERR_ARG, "a non-periodic z needs nz >= 8 (nz=%d)", p->nz);
+    if (p->kernel == 2) return fail(PCMX_ERR_ARG, "kernel 2 runs periodic z only");
+    for (int v = 0; v < p->mvar; ++v)
+      for (int c : {p->bcz_bot[v], p->bcz_top[v]})
+        if (c != PCMX_BC_S && c != PCMX_BC_A && c != PCMX_BC_A2)
+          return fail(PCMX_ERR_ARG, "bcz of variable %d: code %d is not PCMX_BC_S, _A or _A2", v, c);
+  }
   if (p->kernel < 0 || p->kernel > 3) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
   if ((p->lupw_uu != 0 && p->lupw_uu != 1) || (p->lupw_lnrho != 0 && p->lupw_lnrho != 1))
     return fail(PCMX_ERR_ARG, "lupw_uu and lupw_lnrho are 0 or 1");
@@ -109,6 +119,7 @@ Phys make_phys(const pcmx_params& p) {
   P.cdt = p.cdt;
   P.cdtv = p.cdtv;
   P.polytropic = (p.iss < 0 && p.gamma != 1.0) ? 1 : 0;
+  P.gravz = p.gravz;
   return P;
 }
 
@@ -148,6 +159,9 @@ struct Engine {
   // helical forcing added after each step (pcmx_set_forcing)
   pcmx_forcing frc{};
   double2* ftab = nullptr;    // nx + ny + nz unit phasors
+  // non-periodic z: ghost planes of the state ([v][6][ny][nx]) and the rules
+  double* zgh = nullptr;
+  ZBC zbc{};
 };
 
 Engine* g = nullptr;
@@ -190,7 +204,20 @@ KArgs base_args(Engine& e) {
   a.w = e.w;
   a.wout = e.w;
   a.means = e.results + R_MEAN0;
+  a.zgh = e.zgh;
   return a;
+}
+
+// boundconds_z of a non-periodic z on state f: ghost planes into e.zgh and,
+// with zero_a, the boundary planes of 'a' variables set to 0 (before an RHS,
+// as the reference's pde does, equ.f90:209-224).  No-op for periodic z.
+int apply_zbc(Engine& e, const double* f, bool zero_a) {
+  if (!e.zgh) return PCMX_OK;
+  const long nxy = (long)e.par.nx * e.par.ny;
+  hipLaunchKernelGGL(k_zbc, dim3((unsigned)((nxy + 255) / 256), e.nvar), dim3(256), 0, e.stream,
+                     const_cast<double*>(f), e.zgh, e.zbc, e.nvar, nxy, e.par.nz, zero_a ? 1 : 0);
+  HIPCHECK(hipGetLastError());
+  return PCMX_OK;
 }
 
 int finalize_red(Engine& e, bool set_dt, bool set_means) {
@@ -222,6 +249,7 @@ void fuse_forcing(const Engine& e, KArgs& a, double dt) {
 int substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   KArgs a = base_args(e);
   a.fin = e.f[e.cur];
+  if (int rc = apply_zbc(e, a.fin, true)) return rc;
   a.fout = e.f[1 - e.cur];
   a.first = (s == 0);
   a.alpha = ALPHA_TS[s];
@@ -320,8 +348,11 @@ int add_forcing(Engine& e, double dt) {
 }
 
 int run_substep(Engine& e, int s, double dt_host, bool diag, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
-  const int rc = (e.kernel == 3 && e.par.ldt) ? substep_deferred(e, s, dt_host, diag, ev0, ev1)
-                                              : substep(e, s, dt_host, diag, ev0, ev1);
+  // the deferred first-substep update forms f1 while staging, which the
+  // boundary conditions of a non-periodic z would have to follow: with one,
+  // Courant steps take the explicit update
+  const int rc = (e.kernel == 3 && e.par.ldt && e.par.lperi_z) ? substep_deferred(e, s, dt_host, diag, ev0, ev1)
+                                                               : substep(e, s, dt_host, diag, ev0, ev1);
   if (rc || s != 2 || !e.frc.active || forcing_fused(e)) return rc;
   return add_forcing(e, dt_host);
 }
@@ -343,6 +374,7 @@ void free_engine(Engine* e) {
   for (double* p : {e->f[0], e->f[1], e->w, e->w2, e->partials, e->results, e->staging})
     if (p) (void)hipFree(p);
   if (e->ftab) (void)hipFree(e->ftab);
+  if (e->zgh) (void)hipFree(e->zgh);
   for (int k = 0; k < 2; ++k) {
     if (e->pin[k]) (void)hipHostFree(e->pin[k]);
     if (e->pin_ev[k]) (void)hipEventDestroy(e->pin_ev[k]);
@@ -462,9 +494,14 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   if ((rc = alloc(&e->f[0], bytes)) || (rc = alloc(&e->f[1], bytes)) || (rc = alloc(&e->w, bytes)) ||
       (rc = alloc(&e->partials, sizeof(double) * NRED * e->partials_cap)) ||
       (rc = alloc(&e->results, sizeof(double) * R_COUNT)) ||
-      (e->kernel == 3 && par->ldt && (rc = alloc(&e->w2, bytes)))) {
+      (e->kernel == 3 && par->ldt && (rc = alloc(&e->w2, bytes))) ||
+      (!par->lperi_z && (rc = alloc(&e->zgh, sizeof(double) * 6 * par->nx * par->ny * e->nvar)))) {
     free_engine(e);
     return rc;
+  }
+  for (int v = 0; v < 8; ++v) {
+    e->zbc.bot[v] = par->bcz_bot[v];
+    e->zbc.top[v] = par->bcz_top[v];
   }
   hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (err != hipSuccess) {
@@ -536,9 +573,11 @@ int pcmx_copy_farray(double* f, int mfarray) {
   const long mw = mw_of(e.par);
   if (!e.staging) HIPCHECK(hipMalloc((void**)&e.staging, sizeof(double) * mw));
   const double* state = current_state(e);
+  if ((rc = apply_zbc(e, state, false))) return rc;  // z ghosts of the current state
+  const long nxy = (long)e.par.nx * e.par.ny;
   for (int v = 0; v < e.nvar; ++v) {
     hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, state + (long)v * e.nxyz,
-                       e.staging, e.par.nx, e.par.ny, e.par.nz);
+                       e.staging, e.par.nx, e.par.ny, e.par.nz, e.zgh ? e.zgh + v * 6 * nxy : nullptr);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(f + (long)v * mw, e.staging, sizeof(double) * mw, hipMemcpyDeviceToHost, e.stream));
     HIPCHECK(hipStreamSynchronize(e.stream));
@@ -589,6 +628,7 @@ int pcmx_prepare_for_first_substep(double t, double* dt1) {
   Engine& e = *g;
   KArgs a = base_args(e);
   a.fin = current_state(e);
+  if ((rc = apply_zbc(e, a.fin, true))) return rc;
   a.first = 1;
   a.mode = MODE_DT;
   a.stage = ST_DT;
@@ -669,7 +709,7 @@ int pcmx_test_bcs(double* maxdiff) {
     return fail(PCMX_ERR_HIP, "hipMalloc failed in pcmx_test_bcs");
   }
   (void)hipMemcpy(d_in, in.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, d_in, d_out, p.nx, p.ny, p.nz);
+  hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, d_in, d_out, p.nx, p.ny, p.nz, nullptr);
   (void)hipStreamSynchronize(e.stream);
   (void)hipMemcpy(out.data(), d_out, sizeof(double) * mw, hipMemcpyDeviceToHost);
   (void)hipFree(d_in);
@@ -723,7 +763,9 @@ int pcmx_tune(double* best_ms) {
   // buffer and the 2N registers, which hold nothing between steps, so the
   // state is untouched.
   if (e.pending_f1) return fail(PCMX_ERR_STATE, "pcmx_tune between substeps 1 and 2 of a step");
+  if ((rc = apply_zbc(e, e.f[e.cur], false))) return rc;
   const bool ldt = e.par.ldt != 0;
+  const bool deferred = ldt && e.par.lperi_z;  // run_substep's choice
   KArgs st[3];
   for (int s = 0; s < 3; ++s) {
     KArgs& a = st[s];
@@ -745,6 +787,8 @@ int pcmx_tune(double* best_ms) {
     st[0].stage = ST_W;
     st[0].mode = MODE_W;
     st[0].fout = nullptr;
+  }
+  if (deferred) {
     st[1].stage = ST_S2D;
     st[1].wout = e.w2;
     st[2].w = e.w2;
@@ -895,7 +939,7 @@ int pcmx_set_forcing(const pcmx_forcing* frc) {
       double c;
       {
 #pragma clang fp contract(off)
-        c = frc->xyz0[a] + (l + 0.5) * d[a];
+        c = frc->xyz0[a] + (l + ((a == 2 && !p.lperi_z) ? 0.0 : 0.5)) * d[a];
       }
       const double th = frc->kk[a] * frc->k1_ff * c + (a == 0 ? frc->phase : 0.0);
       tab[o++] = make_double2(std::cos(th), std::sin(th));
@@ -955,6 +999,8 @@ int pcmx_write_snapshot(const char* path, double t, const double* xyz0) {
   // record 1: f(mx,my,mz,1:mvar) with periodic ghosts (io_dist.f90:187)
   FRecord rec(fp, (uint64_t)mw * e.nvar * sizeof(double), 0);
   const double* state = current_state(e);
+  if ((rc = apply_zbc(e, state, false))) return rc;  // z ghosts of the current state
+  const long nxy = (long)p.nx * p.ny;
   const long nch = (mw + CHUNK - 1) / CHUNK;  // chunks per variable
   const long nall = nch * e.nvar;
   auto issue = [&](long c) -> int {  // pack (first chunk of a variable) and copy chunk c
@@ -962,7 +1008,7 @@ int pcmx_write_snapshot(const char* path, double t, const double* xyz0) {
     const long off = (c % nch) * CHUNK, n = std::min(CHUNK, mw - off);
     if (c % nch == 0) {
       hipLaunchKernelGGL(k_pack_ghosts, dim3(1024), dim3(256), 0, e.stream, state + (long)v * e.nxyz, e.staging,
-                         p.nx, p.ny, p.nz);
+                         p.nx, p.ny, p.nz, e.zgh ? e.zgh + v * 6 * nxy : nullptr);
       HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipMemcpyAsync(e.pin[c & 1], e.staging + off, sizeof(double) * n, hipMemcpyDeviceToHost, e.stream));
@@ -988,13 +1034,14 @@ int pcmx_write_snapshot(const char* path, double t, const double* xyz0) {
   std::vector<double> grid;
   grid.reserve(4 + mx + my + mz);
   grid.push_back(t);
-  auto coord = [](double x0, int i, double d) {
+  auto coord = [](double x0, int i, double d, double o) {
 #pragma clang fp contract(off)
-    return x0 + (i - 3.5) * d;  // a rounded product, then the sum (no fused multiply-add)
+    return x0 + (i - o) * d;  // a rounded product, then the sum (no fused multiply-add)
   };
-  for (int i = 1; i <= mx; ++i) grid.push_back(coord(xyz0[0], i, p.dx));
-  for (int i = 1; i <= my; ++i) grid.push_back(coord(xyz0[1], i, p.dy));
-  for (int i = 1; i <= mz; ++i) grid.push_back(coord(xyz0[2], i, p.dz));
+  // periodic: cell centres, x(l1) = x0 + dx/2; non-periodic z: z(n1) = z0
+  for (int i = 1; i <= mx; ++i) grid.push_back(coord(xyz0[0], i, p.dx, 3.5));
+  for (int i = 1; i <= my; ++i) grid.push_back(coord(xyz0[1], i, p.dy, 3.5));
+  for (int i = 1; i <= mz; ++i) grid.push_back(coord(xyz0[2], i, p.dz, p.lperi_z ? 3.5 : 4.0));
   grid.push_back(p.dx);
   grid.push_back(p.dy);
   grid.push_back(p.dz);

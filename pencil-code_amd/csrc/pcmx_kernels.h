@@ -45,7 +45,20 @@ struct KArgs {
   const double2* ftab;
   double ffact;
   int ftarget;
+  // non-periodic z: ghost planes of fin ([var][6][ny][nx], planes -3..-1 then
+  // nz..nz+2, filled by k_zbc before the launch), or nullptr for periodic z
+  const double* zgh;
 };
+
+// Source of z plane p of variable v (p in -3 .. nz+2): the state, with
+// periodic wrap, or the ghost-plane buffer of a non-periodic z.  Uniform
+// per plane.
+__device__ __forceinline__ const double* zsrc(const KArgs& a, int v, int p) {
+  const long nxy = (long)a.nx * a.ny;
+  if (a.zgh && (p < 0 || p >= a.nz)) return a.zgh + ((long)v * 6 + (p < 0 ? p + 3 : p - a.nz + 3)) * nxy;
+  const int pz = p < 0 ? p + a.nz : (p >= a.nz ? p - a.nz : p);
+  return a.fin + (long)v * a.nxyz + (long)pz * nxy;
+}
 
 // Helical forcing at cell (l, m, n) (forcing_hel, forcing.f90:2066-2474,
 // default path: force_ampl = 1, cos(omega_ff t) = 1, force1_scl = 1).  tab

@@ -30,6 +30,7 @@ struct Phys {
   double nu, eta, chi, cp, mu0, mu01;
   double cs20, gamma, gamma_m1, gamma1, cv1, lnrho0, lnTT0;
   double cdt, cdtv;
+  double gravz;     // constant vertical gravity (gravity_simple, gravz_profile='const'), or 0
   int polytropic;   // noentropy with gamma /= 1 (ilnrho_ss, leos_isentropic)
 };
 
@@ -124,6 +125,46 @@ __device__ __forceinline__ T d_sij2(const T (&s)[3][3]) {
 }
 
 // ---------------------------------------------------------------------------
+// The pencils of the Courant condition (set_dt1_max, equ.f90:1858-1912),
+// shared by the RHS kernels and the dt pass of Courant steps so that both
+// evaluate them with the same arithmetic.
+
+// Sound speed squared (eos_idealgas.f90:871, 969-1000).
+template <typename T, bool ENT, bool POLY>
+__device__ __forceinline__ T eos_cs2(T lnrho, T ss, const Phys& P) {
+  if constexpr (ENT) return (T)P.cs20 * pexp<T>((T)P.cv1 * ss + (T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
+  else if constexpr (POLY) return (T)P.cs20 * pexp<T>((T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
+  else return (T)P.cs20;
+}
+
+// advec_uu (hydro.f90:3500).
+template <typename T>
+__device__ __forceinline__ T advec_uu_of(T u0, T u1, T u2, const Phys& P) {
+  return fabs(u0) * (T)P.dx_1 + fabs(u1) * (T)P.dy_1 + fabs(u2) * (T)P.dz_1;
+}
+
+// advec_va2 (magnetic.f90:5180).
+template <typename T>
+__device__ __forceinline__ T advec_va2_of(const T (&bb)[3], T rho1, const Phys& P) {
+  const T b1x = bb[0] * (T)P.dx_1, b1y = bb[1] * (T)P.dy_1, b1z = bb[2] * (T)P.dz_1;
+  return (b1x * b1x + b1y * b1y + b1z * b1z) * (T)P.mu01 * rho1;
+}
+
+// dt1_max of one cell from the advective pencils (set_dt1_max).
+template <typename T, bool MAG, bool ENT>
+__device__ __forceinline__ double courant_dt1(T advec_uu, T cs2, T advec_va2, const Phys& P) {
+  const T advec_cs2 = cs2 * (T)P.dxyz_2;
+  const T advec2 = (MAG ? advec_va2 : (T)0) + advec_cs2;
+  const T maxadvec = advec_uu + psqrt<T>(advec2);
+  T maxdiffus = (T)P.nu * (T)P.dxyz_2;                                 // viscosity.f90:2435
+  if constexpr (MAG) maxdiffus = fmax(maxdiffus, (T)P.eta * (T)P.dxyz_2);
+  if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
+  const T dt1_advec = maxadvec / (T)P.cdt;
+  const T dt1_diffus = maxdiffus / (T)P.cdtv;
+  return (double)psqrt<T>(dt1_advec * dt1_advec + dt1_diffus * dt1_diffus);
+}
+
+// ---------------------------------------------------------------------------
 // The RHS is split in two halves so that a kernel may evaluate them on
 // different threads (the field half and the flow half exchange four numbers
 // per cell): field_point covers src/magnetic.f90 (B, J, J x B / rho,
@@ -181,10 +222,7 @@ __device__ __forceinline__ void field_point(const Derivs<T>& d, T rho1, const Ph
 #pragma unroll
   for (int i = 0; i < 3; ++i) o.jxbr[i] = rho1 * jxb[i];
   o.advec_va2 = 0;
-  if constexpr (DT) {
-    const T b1x = bb[0] * (T)P.dx_1, b1y = bb[1] * (T)P.dy_1, b1z = bb[2] * (T)P.dz_1;
-    o.advec_va2 = (b1x * b1x + b1y * b1y + b1z * b1z) * (T)P.mu01 * rho1;  // :5180
-  }
+  if constexpr (DT) o.advec_va2 = advec_va2_of<T>(bb, rho1, P);
 #pragma unroll
   for (int j = 0; j < 3; ++j) wa[j] = wa[j] + (double)(uxb[j] + (T)P.eta * del2a[j]);  // :6247,6524
   *b2_out = (double)b2;
@@ -237,8 +275,7 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
   T cs2, TT1 = 0, glnTT[3] = {0, 0, 0}, del2lnTT = 0, ugss = 0;
   T fpres[3];
   if constexpr (!ENT) {
-    if constexpr (POLY) cs2 = (T)P.cs20 * pexp<T>((T)P.gamma_m1 * (lnrho - (T)P.lnrho0));
-    else cs2 = (T)P.cs20;                                               // eos_idealgas.f90:969-1000
+    cs2 = eos_cs2<T, false, POLY>(lnrho, (T)0, P);
 #pragma unroll
     for (int j = 0; j < 3; ++j) fpres[j] = -cs2 * d.glnrho[j];          // noentropy.f90:252-300
   } else {
@@ -246,7 +283,7 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
     const T del2lnrho = d.del2lnrho;
     const T del2ss = d.del2ss;
     const T dl = lnrho - (T)P.lnrho0;
-    cs2 = (T)P.cs20 * pexp<T>((T)P.cv1 * ss + (T)P.gamma_m1 * dl);       // eos_idealgas.f90:871
+    cs2 = eos_cs2<T, true, false>(lnrho, ss, P);
     const T lnTT = (T)P.lnTT0 + (T)P.cv1 * ss + (T)P.gamma_m1 * dl;      // :876
     TT1 = pexp<T>(-lnTT);
 #pragma unroll
@@ -291,8 +328,7 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
   k.TT1 = TT1;
   k.cs2 = cs2;
   k.advec_uu = 0;
-  if constexpr (DT)
-    k.advec_uu = fabs(u0) * (T)P.dx_1 + fabs(u1) * (T)P.dy_1 + fabs(u2v) * (T)P.dz_1;  // hydro.f90:3500
+  if constexpr (DT) k.advec_uu = advec_uu_of<T>(u0, u1, u2v, P);
   *u2_out = (double)u2;
 }
 
@@ -307,17 +343,10 @@ __device__ __forceinline__ void flow_point_post(const FlowKeep<T>& k, const Fiel
     for (int j = 0; j < 3; ++j) wf[j] = wf[j] + (double)fo.jxbr[j];
     if constexpr (ENT) wf[4] = wf[4] + (double)((T)P.eta * (T)P.mu0 * fo.j2 * k.rho1 * k.TT1);
   }
-  if constexpr (DT) {
-    const T advec_cs2 = k.cs2 * (T)P.dxyz_2;
-    const T advec2 = (MAG ? fo.advec_va2 : (T)0) + advec_cs2;
-    const T maxadvec = k.advec_uu + psqrt<T>(advec2);
-    T maxdiffus = (T)P.nu * (T)P.dxyz_2;                                 // viscosity.f90:2435
-    if constexpr (MAG) maxdiffus = fmax(maxdiffus, (T)P.eta * (T)P.dxyz_2);
-    if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
-    const T dt1_advec = maxadvec / (T)P.cdt;
-    const T dt1_diffus = maxdiffus / (T)P.cdtv;
-    *dt1 = (double)psqrt<T>(dt1_advec * dt1_advec + dt1_diffus * dt1_diffus);
-  }
+  // addgravity after the magnetic terms (equ.f90:1316; gravity_simple.f90:1083-1162);
+  // adding 0 without gravity leaves the value unchanged
+  wf[2] = wf[2] + P.gravz;
+  if constexpr (DT) *dt1 = courant_dt1<T, MAG, ENT>(k.advec_uu, k.cs2, fo.advec_va2, P);
 }
 
 // Whole RHS of one cell: adds it to w[] (already alpha-scaled; slots 0..2 uu,

@@ -293,17 +293,21 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   const double dtb1 = a.d_dt ? a.d_dt[0] * a.beta1 : a.dtb1;
   constexpr int NW = G::FORM1 ? NV * K : 1;
   double pre[NV * K], prem[2] = {0.0, 0.0}, prw[NW], prwm[2] = {0.0, 0.0};
+  // staged planes beyond a non-periodic z boundary come from the ghost-plane
+  // buffer (variable stride 6 planes); the deferred-update stage S2D is not
+  // run with one
   auto load_plane = [&](int p) {
-    const double* src = a.fin + zoff(p);
+    const double* src = zsrc(a, 0, p);
+    const long vs = zsrc(a, 1, p) - src;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      const double* sv = src + v * nxyz;
+      const double* sv = src + v * vs;
 #pragma unroll
       for (int k = 0; k < K; ++k) pre[v * K + k] = sv[poff[k]];
     }
     if (has_rem) {
-      prem[0] = src[rva * nxyz + roff];
-      prem[1] = src[rvb * nxyz + roff];
+      prem[0] = src[rva * vs + roff];
+      prem[1] = src[rvb * vs + roff];
     }
     if constexpr (G::FORM1) {
       const double* srw = a.w + zoff(p);

@@ -133,6 +133,10 @@ contains
       call density_pushpars2c(p_par)
       call c_f_pointer(transfer(p_par(5), c_null_ptr), lupw)
       par%lupw_lnrho = merge(1, 0, lupw)
+      par%gravz = 0.
+      par%lperi_z = 1
+      par%bcz_bot = 0
+      par%bcz_top = 0
 !
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)

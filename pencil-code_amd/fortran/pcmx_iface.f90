@@ -25,6 +25,9 @@ module pcmx_iface
     real(c_double) :: cdt, cdtv
     integer(c_int) :: ldt, precision, kernel
     integer(c_int) :: lupw_uu, lupw_lnrho
+    real(c_double) :: gravz
+    integer(c_int) :: lperi_z
+    integer(c_int) :: bcz_bot(8), bcz_top(8)
   end type pcmx_params
 
   ! pcmx_forcing of include/pcmx.h: helical forcing after each step

@@ -49,7 +49,24 @@ class Params(ctypes.Structure):
         ("cdt", ctypes.c_double), ("cdtv", ctypes.c_double),
         ("ldt", ctypes.c_int), ("precision", ctypes.c_int), ("kernel", ctypes.c_int),
         ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
+        ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
+        ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
     ]
+
+
+BC_CODES = {"s": 1, "a": 2, "a2": 3}  # PCMX_BC_S, _A, _A2
+
+
+def bc_codes(bcz, nv):
+    """Per-variable (bottom, top) code arrays from run.in style entries:
+    's', 'a', 'a2', or 'bottom:top' such as 'a2:s'."""
+    if len(bcz) != nv:
+        raise ValueError(f"{len(bcz)} z boundary conditions for {nv} variables")
+    bot, top = (ctypes.c_int * 8)(), (ctypes.c_int * 8)()
+    for k, b in enumerate(bcz):
+        lo, _, hi = b.partition(":")
+        bot[k], top[k] = BC_CODES[lo], BC_CODES[hi or lo]
+    return bot, top
 
 
 class Box(ctypes.Structure):
@@ -77,13 +94,15 @@ ENS_NO_GRAPH = 1
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
                 cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
-                kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False):
+                kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None):
     """Parameters of one periodic box with the reference defaults
     (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
     registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
     kernel None takes PCMX_KERNEL from the environment (0 auto, 1 direct,
     2 z-marching, 3 z-marching with the flow and field halves on separate
-    wavefronts), default auto."""
+    wavefronts), default auto.  bcz: None for a periodic z, else one run.in
+    entry per variable ('s', 'a', 'a2', 'bottom:top'); the z spacing is then
+    Lz/(nz-1), boundary points on the first and last plane."""
     if kernel is None:
         kernel = int(os.environ.get("PCMX_KERNEL", KERNEL_AUTO))
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
@@ -91,8 +110,11 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, c
     iss = 4 if lentropy else -1
     iax = (5 if lentropy else 4) if lmagnetic else -1
     mvar = 4 + (1 if lentropy else 0) + (3 if lmagnetic else 0)
-    return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, Lz / nz, nu, eta, chi, cs0, gamma,
-                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho))
+    bot, top = bc_codes(bcz, mvar) if bcz is not None else ((ctypes.c_int * 8)(), (ctypes.c_int * 8)())
+    dz = Lz / nz if bcz is None else Lz / (nz - 1)
+    return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, dz, nu, eta, chi, cs0, gamma,
+                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho),
+                  gravz, int(bcz is None), bot, top)
 
 
 _lib = None
