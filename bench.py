@@ -354,6 +354,41 @@ def run_published(n, steps, warmup, tune=True):
     return out
 
 
+ZBOUND_BC = ["s", "s", "a", "a2", "a", "a", "s"]
+
+
+def run_zbound(n, steps, warmup, tune=True):
+    """Isothermal MHD with non-periodic z boundaries and constant gravity:
+    bcz = s, s, a, a2, a, a, s for (uu, lnrho, aa) (bc_sym_z), gravz = -0.5,
+    box 2pi x 2pi x 2pi with boundary points on the first and last z plane,
+    nu = eta = 1e-3, fixed dt = 1e-3; band-limited uu/aa (seed 7) tapered by
+    sin^2(z/2) so that they meet the boundary conditions, and the hydrostatic
+    lnrho = -z/2.  The same state run periodic in z without gravity is timed
+    beside it."""
+    interior = pcmx_ics.config_state("shells", n, 7, lmagnetic=True)
+    z = np.linspace(0.0, 2 * math.pi, n)[:, None, None]
+    env = np.sin(z / 2) ** 2
+    interior[0:3] *= env
+    interior[4:7] *= env
+    interior[3] = -0.5 * z
+    f = pcmx.farray_from_interior(interior)
+    del interior, env
+    desc = (f"isothermal MHD {n}^3, 7 FP64 vars, z boundaries {','.join(ZBOUND_BC)}, gravz=-0.5, nu=eta=1e-3, "
+            "fixed dt=1e-3, band-limited uu/aa (seed 7) tapered by sin^2(z/2), lnrho=-z/2")
+    log(f"bench: z boundaries + gravity: {desc}")
+    out = {"workload": desc, "unit": "Mcell-updates/s", "grid": n}
+    for name, prec, zb in (("fp64", pcmx.PREC_FP64, True), ("mixed", pcmx.PREC_MIXED, True),
+                           ("fp64_periodic_same_state", pcmx.PREC_FP64, False)):
+        par = pcmx.make_params(n, lmagnetic=True, nu=1e-3, eta=1e-3, precision=prec,
+                               gravz=-0.5 if zb else 0.0, bcz=ZBOUND_BC if zb else None)
+        r = run_form(f, par, steps, warmup, 1e-3, tune=tune)
+        out[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms_per_step"], 4),
+                     "kernel_ms": round(r["kernel_ms"], 4), "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                     "urms": r["urms"], "plan": r["plan"]}
+        log(f"bench: z boundaries {name}: {out[name]}")
+    return out
+
+
 ENSEMBLE_DESC = ("ensemble of 64 isothermal MHD boxes, sizes uniform in {32,48,64,96}^3 (side 2pi), "
                  "eta log-uniform in [1e-3,1e-2], Pm=nu/eta log-uniform in [0.5,10], band-limited uu/aa with "
                  "distinct seeds, fixed dt per box, 200 steps; mixed form = FP64 state with FP32 RHS")
@@ -452,6 +487,8 @@ def main():
     ap.add_argument("--published", type=int, default=0, metavar="N",
                     help="time the physics of the reference's published CPU timings (upwinding, helical "
                          "forcing of A, Courant dt) at N^3")
+    ap.add_argument("--zbound", type=int, default=0, metavar="N",
+                    help="time isothermal MHD at N^3 with non-periodic z boundaries and gravity")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run 4 device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -462,6 +499,9 @@ def main():
     threads = min(16, os.cpu_count() or 1)
     if args.io:
         print(json.dumps(run_io(args.io)), flush=True)
+        return
+    if args.zbound:
+        print(json.dumps(run_zbound(args.zbound, args.steps, args.warmup, tune=not args.no_tune)), flush=True)
         return
     if args.published:
         print(json.dumps(run_published(args.published, args.steps, args.warmup, tune=not args.no_tune)),

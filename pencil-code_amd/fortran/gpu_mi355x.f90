@@ -75,13 +75,15 @@ contains
 !
       type(pcmx_params) :: par
       real, pointer :: nu_p, eta_p, chi_p, cs20_p, gamma_p, cp_p
+      real, dimension(:), pointer :: gz_p
       integer(KIND=ikind8), dimension(200) :: p_par
       logical, pointer :: lupw
       character(len=512) :: str
+      integer :: j, k
 !
       str=''
       if (ncpus /= 1) str=trim(str)//', ncpus>1'
-      if (.not. all(lperi)) str=trim(str)//', non-periodic boundaries'
+      if (.not. (lperi(1) .and. lperi(2))) str=trim(str)//', non-periodic x or y'
       if (.not. lcartesian_coords) str=trim(str)//', non-Cartesian coordinates'
       if (.not. all(lequidist)) str=trim(str)//', non-equidistant grid'
       if (.not. (lhydro .and. ldensity)) str=trim(str)//', needs hydro and density'
@@ -133,10 +135,38 @@ contains
       call density_pushpars2c(p_par)
       call c_f_pointer(transfer(p_par(5), c_null_ptr), lupw)
       par%lupw_lnrho = merge(1, 0, lupw)
+!
+!  Constant vertical gravity (gravity_simple, gravz_profile='const'): the
+!  profile the Gravity module shares (gravity_simple.f90:255, :516-523).
+!
       par%gravz = 0.
-      par%lperi_z = 1
+      if (lgrav) then
+        call get_shared_variable('gravz_zpencil', gz_p)
+        if (any(gz_p /= gz_p(n1))) &
+            call fatal_error('initialize_GPU', 'only gravz_profile=''const'' runs on the GPU')
+        par%gravz = gz_p(n1)
+      endif
+!
+!  z boundaries: periodic, or per variable 's', 'a', 'a2' below and above
+!  (bcz12, split by the reference from run_pars bcz; bc_sym_z).
+!
+      par%lperi_z = merge(1, 0, lperi(3))
       par%bcz_bot = 0
       par%bcz_top = 0
+      if (.not. lperi(3)) then
+        do j = 1, mvar
+          do k = 1, 2
+            select case (trim(bcz12(j,k)))
+            case ('s'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_S; else; par%bcz_top(j) = PCMX_BC_S; endif
+            case ('a'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_A; else; par%bcz_top(j) = PCMX_BC_A; endif
+            case ('a2'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_A2; else; par%bcz_top(j) = PCMX_BC_A2; endif
+            case default
+              call fatal_error('initialize_GPU', 'bcz='''//trim(bcz12(j,k))// &
+                  ''' is not provided on the GPU (s, a, a2)')
+            endselect
+          enddo
+        enddo
+      endif
 !
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)

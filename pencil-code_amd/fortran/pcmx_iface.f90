@@ -30,6 +30,9 @@ module pcmx_iface
     integer(c_int) :: bcz_bot(8), bcz_top(8)
   end type pcmx_params
 
+  ! z boundary codes of pcmx_params (PCMX_BC_* of include/pcmx.h)
+  integer(c_int), parameter :: PCMX_BC_S = 1, PCMX_BC_A = 2, PCMX_BC_A2 = 3
+
   ! pcmx_forcing of include/pcmx.h: helical forcing after each step
   type, bind(C) :: pcmx_forcing
     integer(c_int) :: active, target, scale_sqrt_dt, reserved

@@ -8,7 +8,8 @@ module Cparam
   integer, parameter :: nx=32, ny=32, nz=32, mx=nx+6, my=ny+6, mz=nz+6
   integer, parameter :: mvar=7, mfarray=7, ncpus=1
   logical, parameter :: lhydro=.true., ldensity=.true., lmagnetic=.true., lentropy=.false.
-  logical, parameter :: ltemperature=.false., lthermal_energy=.false., lforcing=.true.
+  logical, parameter :: ltemperature=.false., lthermal_energy=.false., lforcing=.true., lgrav=.true.
+  integer, parameter :: mcom=mvar, bclen=3
 end module Cparam
 module Cdata
   use Cparam
@@ -20,6 +21,8 @@ module Cdata
   logical, dimension(3) :: lperi=.true., lequidist=.true.
   integer :: iux=1, ilnrho=4, iss=0, iax=5, itorder=3, nt=10
   real, dimension(3) :: xyz0=-3.14159
+  integer :: n1=4
+  character(len=bclen), dimension(mcom,2) :: bcz12=''
 end module Cdata
 module General
   implicit none
@@ -43,10 +46,17 @@ contains
 end module Messages
 module SharedVariables
   implicit none
+  interface get_shared_variable
+    module procedure get_r0, get_r1
+  end interface
 contains
-  subroutine get_shared_variable(name, p)
+  subroutine get_r0(name, p)
     character(len=*) :: name
     real, pointer :: p
+  end subroutine
+  subroutine get_r1(name, p)
+    character(len=*) :: name
+    real, dimension(:), pointer :: p
   end subroutine
 end module SharedVariables
 module EquationOfState
