@@ -192,3 +192,43 @@ def test_update_on_gpu_by_name_changes_viscosity():
     op.nu = 2e-2
     o1, _, _, _ = O.advance(f0, op, 2)
     assert np.all(rel_linf(pcmx.interior_of(out, 4), o1) <= FP64_1STEP)
+
+
+def _steps(f0, gp, nsteps, dt, timing, calls=1, change=None, forcing=None):
+    f = pcmx.farray_from_interior(f0)
+    pcmx.initialize_gpu(f, gp, nt=1)
+    try:
+        if forcing:
+            pcmx.set_forcing(**forcing)
+        pcmx.set_timing(timing)
+        t = 0.0
+        for k in range(calls):
+            if change and k == 1:
+                change()
+            t, _ = pcmx.time_step(nsteps, t, dt, ldiag=True)
+        red = pcmx.get_gpu_reduced_vars()
+        out = np.zeros_like(f)
+        pcmx.copy_farray_from_gpu(out)
+    finally:
+        pcmx.finalize_gpu()
+    return pcmx.interior_of(out, gp.mvar).copy(), t, red
+
+
+@pytest.mark.parametrize("name,nsteps", [("hydro32", 6), ("mhd32", 5), ("mhd_ragged", 4)])
+def test_graph_replay_matches_plain_launches(name, nsteps):
+    """Fixed steps replay captured pairs of steps (kernel timing off); the
+    state, t and diagnostics equal those of plain launches bitwise, across
+    repeated replays, an odd step count and a recapture after a parameter
+    change and a new dt."""
+    c, gp, f0 = setup_case(name)
+    announce(f"test_graph_replay_matches_plain_launches[{name}] {nsteps} steps x 3 calls")
+    change = lambda: pcmx.update_on_gpu(-1, "nu", 2 * c["nu"])  # noqa: E731
+    a = _steps(f0, gp, nsteps, c["dt"], timing=False, calls=3, change=change)
+    b = _steps(f0, gp, nsteps, c["dt"], timing=True, calls=3, change=change)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[1] == b[1]
+    np.testing.assert_array_equal(a[2], b[2])
+    # a different dt recaptures
+    a2 = _steps(f0, gp, nsteps, 0.5 * c["dt"], timing=False)
+    b2 = _steps(f0, gp, nsteps, 0.5 * c["dt"], timing=True)
+    np.testing.assert_array_equal(a2[0], b2[0])

@@ -207,3 +207,16 @@ def test_set_forcing_errors_and_switch_off():
         pcmx.finalize_gpu()
     o, _, _, _ = O.advance(f0, oracle_params(gp, c["dt"]), 1)
     assert np.all(rel_linf(pcmx.interior_of(out, 4), o) <= FP64_1STEP)
+
+
+def test_graph_replay_with_upwinding_and_forcing():
+    """Captured step pairs (kernel timing off) against plain launches, with
+    upwinding and the fused forcing epilogue: bitwise equal."""
+    from test_gpu_parity import _steps
+    announce("test_graph_replay_with_upwinding_and_forcing: mhd32, kernel 3, 6 steps")
+    c, gp, f0 = setup_case("mhd32", kernel=3, lupw_uu=True, lupw_lnrho=True)
+    kw, _ = forcing_pair((1, -2, 1), gp, force=0.1, target=gp.iax)
+    a = _steps(f0, gp, 6, c["dt"], timing=False, calls=2, forcing=kw)
+    b = _steps(f0, gp, 6, c["dt"], timing=True, calls=2, forcing=kw)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[1] == b[1]

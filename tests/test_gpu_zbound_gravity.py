@@ -137,3 +137,15 @@ def test_rejected_configurations():
     bad.bcz_top[2] = 7
     with pytest.raises(pcmx.PcmxError, match="code 7"):
         pcmx.initialize_gpu(None, bad, nt=0)
+
+
+def test_graph_replay_with_z_boundaries():
+    from test_gpu_parity import _steps
+    n = (32, 16, 24)
+    announce("test_graph_replay_with_z_boundaries: mhd, kernel 3, 6 steps")
+    gp = pcmx.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, kernel=3, L=(2 * math.pi, math.pi, 1.0),
+                          gravz=-1.0, bcz=MHD_BC)
+    f0 = state(n, True, False, 4, -1.0)
+    a = _steps(f0, gp, 6, 1e-3, timing=False, calls=2)
+    b = _steps(f0, gp, 6, 1e-3, timing=True, calls=2)
+    np.testing.assert_array_equal(a[0], b[0])
