@@ -149,3 +149,59 @@ def test_graph_replay_with_z_boundaries():
     a = _steps(f0, gp, 6, 1e-3, timing=False, calls=2)
     b = _steps(f0, gp, 6, 1e-3, timing=True, calls=2)
     np.testing.assert_array_equal(a[0], b[0])
+
+
+@pytest.mark.parametrize("kernel", [1, 3])
+def test_forcing_with_z_boundaries(kernel):
+    """Helical forcing on a grid whose z points include the boundaries
+    (z = z0 + n dz): the phasors follow that grid."""
+    import pcmx_forcing as PF
+    n = (32, 16, 24)
+    announce(f"test_forcing_with_z_boundaries[kernel {kernel}]")
+    L = (2 * math.pi, math.pi, 1.0)
+    gp = pcmx.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, kernel=kernel, L=L, gravz=-1.0, bcz=MHD_BC)
+    op = O.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, dt=1e-3, L=L, gravz=-1.0, bcz=MHD_BC)
+    pars = PF.helical_pars((1, 2, 3), 0.4, 1.3, 0.2, 1e-3, 3.0)
+    xyz0 = (-math.pi, -0.5 * math.pi, -0.5)
+    kw = dict(kk=pars["kk"], phase=pars["phase"], fact=pars["fact"], coef1=pars["coef1"], coef2=pars["coef2"],
+              fda=pars["fda"], k1_ff=1.0, xyz0=xyz0, target=0)
+    ofo = O.make_forcing(pars["kk"], pars["phase"], pars["fact"], pars["coef1"], pars["coef2"], pars["fda"],
+                         k1_ff=1.0, xyz0=xyz0, target=0)
+    f0 = state(n, True, False, 4, -1.0)
+    g, _, _ = pcmx.advance(f0, gp, 5, 1e-3, forcing=kw)
+    o, _, _, _ = O.advance(f0, op, 5, forcing=ofo)
+    err = rel_linf(g, o)
+    announce(f"  rel L-inf per variable: {err}")
+    assert np.all(err <= FP64_1STEP * 10), err
+    plain, _, _, _ = O.advance(f0, op, 5)
+    assert rel_linf(o, plain).max() > 1e-6
+
+
+def test_snapshot_and_dt1_with_z_boundaries(tmp_path):
+    import pcmx_io as I
+    n = (32, 16, 16)
+    announce("test_snapshot_and_dt1_with_z_boundaries")
+    L = (2 * math.pi, math.pi, 1.5)
+    gp = pcmx.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, kernel=3, L=L, gravz=-1.0, bcz=MHD_BC, ldt=True,
+                          cdt=0.4)
+    f0 = state(n, True, False, 6, -1.0)
+    f = pcmx.farray_from_interior(f0)
+    pcmx.initialize_gpu(f, gp, nt=1)
+    try:
+        dt1 = pcmx.gpu_prepare_for_first_substep(0.0)
+        pcmx.time_step(2, 0.0, 0.0)
+        path = tmp_path / "var.dat"
+        xyz0 = (-math.pi, -0.5 * math.pi, -0.75)
+        pcmx.write_snapshot(str(path), 0.0, xyz0)
+        copy = np.zeros_like(f)
+        pcmx.copy_farray_from_gpu(copy)
+    finally:
+        pcmx.finalize_gpu()
+    op = O.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, dt=0.0, L=L, gravz=-1.0, bcz=MHD_BC, cdt=0.4)
+    _, dt1o, _ = O.rhs(f0, op, want_dt=True)
+    assert abs(dt1 - dt1o) <= 1e-13 * dt1o
+    gvar, t, x, y, z, dx, dy, dz = I.read_var(str(path), 7, n)
+    np.testing.assert_array_equal(gvar, copy[:7])
+    # boundary points on the first and last interior z plane
+    assert z[3] == xyz0[2] and abs(z[3 + n[2] - 1] - (xyz0[2] + L[2])) < 1e-14
+    assert dz == gp.dz == L[2] / (n[2] - 1)
