@@ -34,9 +34,10 @@ module GPU
   integer :: gpu_precision=0      !PAR-DOC: 0 FP64, 1 mixed (FP32 stencils on box-mean offsets)
   integer :: gpu_kernel=0         !PAR-DOC: 0 auto, 1 direct, 2 z-marching LDS kernel, 3 split z-marching kernel
   logical :: lgpu_magnetic_forcing=.false. !PAR-DOC: the helical force acts on aa (set with forcing_run_pars lmagnetic_forcing=T)
+  logical :: lgpu_tune=.false.    !PAR-DOC: time the split kernel's tile shapes and z-chunk lengths once at start
 !
   namelist /gpu_run_pars/ ltest_bcs, ltest_rhs, it_test_rhs, gpu_precision, gpu_kernel, &
-                          lgpu_magnetic_forcing
+                          lgpu_magnetic_forcing, lgpu_tune
 !
   real, dimension(n_gpu_reduced_vars) :: reduced_last = 0.
 !
@@ -76,6 +77,7 @@ contains
       type(pcmx_params) :: par
       real, pointer :: nu_p, eta_p, chi_p, cs20_p, gamma_p, cp_p
       real, dimension(:), pointer :: gz_p
+      real(c_double), dimension(3) :: best
       integer(KIND=ikind8), dimension(200) :: p_par
       logical, pointer :: lupw
       character(len=512) :: str
@@ -171,6 +173,15 @@ contains
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)
       if (ltest_bcs) call test_gpu_bcs
+      if (lgpu_tune .and. nt > 0) then
+!  (only the split kernel has plans to choose from; other kernels keep theirs)
+        if (pcmx_tune(best) == 0) then
+          if (lroot) print '(a,f8.3,a,i3,a,i5)', 'initialize_GPU: tuned split kernel: ', best(1), &
+              ' ms per launch, tile width ', nint(best(2)), ', z chunk ', nint(best(3))
+        elseif (lroot) then
+          print '(a)', 'initialize_GPU: no tuning: '//trim(pcmx_error_message())
+        endif
+      endif
 !
     endsubroutine initialize_GPU
 !***********************************************************************

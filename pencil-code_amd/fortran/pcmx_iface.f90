@@ -123,6 +123,12 @@ module pcmx_iface
       real(c_double), value :: t
       real(c_double), dimension(3), intent(in) :: xyz0
     end function
+    ! times the split kernel's tile shapes and z-chunk lengths on the loaded
+    ! state and keeps the fastest: best = ms per launch, tile width, z chunk
+    integer(c_int) function pcmx_tune(best) bind(C, name='pcmx_tune')
+      import :: c_int, c_double
+      real(c_double), dimension(3), intent(out) :: best
+    end function
     integer(c_int) function pcmx_set_forcing(frc) bind(C, name='pcmx_set_forcing')
       import :: c_int, pcmx_forcing
       type(pcmx_forcing), intent(in) :: frc
