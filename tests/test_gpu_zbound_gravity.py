@@ -205,3 +205,17 @@ def test_snapshot_and_dt1_with_z_boundaries(tmp_path):
     # boundary points on the first and last interior z plane
     assert z[3] == xyz0[2] and abs(z[3 + n[2] - 1] - (xyz0[2] + L[2])) < 1e-14
     assert dz == gp.dz == L[2] / (n[2] - 1)
+
+
+def test_per_substep_api_with_z_boundaries_and_courant_dt():
+    """The reference's call sequence (rhs_gpu per substep) equals whole steps,
+    bitwise, with z boundaries, gravity and a Courant dt."""
+    n = (32, 16, 16)
+    announce("test_per_substep_api_with_z_boundaries_and_courant_dt")
+    gp = pcmx.make_params(n, lmagnetic=True, nu=4e-3, eta=3e-3, kernel=3, L=(2 * math.pi, math.pi, 1.0),
+                          gravz=-1.0, bcz=MHD_BC, ldt=True, cdt=0.4)
+    f0 = state(n, True, False, 8, -1.0)
+    a, ta, dta = pcmx.advance(f0, gp, 3, 0.0, per_substep=True)
+    b, tb, dtb = pcmx.advance(f0, gp, 3, 0.0)
+    np.testing.assert_array_equal(a, b)
+    assert ta == tb and dta == dtb
