@@ -66,6 +66,8 @@ typedef struct pcmx_params {
   int lperi_z;               /* init_pars lperi(3): 1 periodic z; 0 the bcz rules below, with
                                 boundary points at the first and last z plane (dz = Lz/(nz-1)) */
   int bcz_bot[8], bcz_top[8];/* run_pars bcz per variable slot when lperi_z == 0: PCMX_BC_* */
+  int lupw_ss;               /* entropy_run_pars lupw_ss: upwinded u.grad ss (entropy.f90:3576);
+                                entropy runs, direct kernel */
 } pcmx_params;
 
 /* z boundary conditions of bc_sym_z (boundcond.f90:3842-3900) */

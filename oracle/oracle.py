@@ -34,6 +34,7 @@ class Params(ctypes.Structure):
         ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
         ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
         ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
+        ("lupw_ss", ctypes.c_int),
     ]
 
 
@@ -68,7 +69,7 @@ def make_forcing(kk, phase, fact, coef1, coef2, fda=(1.0, 1.0, 1.0), k1_ff=1.0, 
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
                 cs0=1.0, gamma=1.0, cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25,
-                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None):
+                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None, lupw_ss=False):
     """Run parameters with the reference defaults (cdata.f90:235-239,
     eos_idealgas.f90:40-46).  bcz: None (periodic z) or one run.in entry per
     variable ('s', 'a', 'a2', or 'bottom:top')."""
@@ -78,7 +79,7 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
     bot, top = bc_codes(bcz, nv) if bcz is not None else ((ctypes.c_int * 8)(), (ctypes.c_int * 8)())
     return Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
                   cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho),
-                  gravz, int(bcz is None), bot, top)
+                  gravz, int(bcz is None), bot, top, int(lupw_ss))
 
 
 def build():

@@ -479,6 +479,8 @@ static void calc_pencils(const ctx* c, const double* f, int m, int n, pencils* p
   } else {
     /* entropy.f90:3554-3670: ugss=u.gss; fpres=-cs2*(glnrho+glnTT)*gamma1 */
     dot_mn(nx, PC(pc, P_U), nx, PC(pc, P_GSS), nx, PC(pc, P_UGSS));
+    /* u_dot_grad(f,iss,gss,uu,ugss,UPWIND=lupw_ss) (:3576) */
+    if (p->lupw_ss) doupwind(g, f, m, n, c->iss, PC(pc, P_U), PC(pc, P_UGSS), PC(pc, P_UPW));
     for (int j = 0; j < 3; ++j)
       for (int l = 0; l < nx; ++l)
         PC(pc, P_FPRES + j)[l] =

@@ -50,6 +50,7 @@ typedef struct {
   double gravz;            /* grav_run_pars gravz with gravz_profile='const' (gravity_simple.f90) */
   int lperi_z;             /* init_pars lperi(3); 0: the bcz rules below hold in z */
   int bcz_bot[8], bcz_top[8]; /* per variable: PCO_BC_S, PCO_BC_A, PCO_BC_A2 (run_pars bcz) */
+  int lupw_ss;             /* entropy_run_pars lupw_ss: upwinded u.grad ss (entropy.f90:3576) */
 } pco_params;
 
 #define PCO_BC_S 1   /* 's':  symmetric about the boundary point */

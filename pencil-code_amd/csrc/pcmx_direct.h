@@ -115,6 +115,8 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
         if (((v < 3) ? (UPW & UPW_UU) : (UPW & UPW_LNRHO)) == 0) continue;
         d.upw[v] = ua[0] * d6(v, 0) + ua[1] * d6(v, 1) + ua[2] * d6(v, 2);
       }
+      if constexpr (ENT && (UPW & UPW_SS) != 0)
+        d.upw[4] = ua[0] * d6(S::ISS, 0) + ua[1] * d6(S::ISS, 1) + ua[2] * d6(S::ISS, 2);
     }
     if constexpr (ENT) {
       d.ss = (T)fc[S::ISS];

@@ -63,8 +63,20 @@ bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t 
     case UPW_UU: launch_direct_dt<T, MAG, ENT, UPW_UU>(c, a, P, s); return true;
     case UPW_LNRHO: launch_direct_dt<T, MAG, ENT, UPW_LNRHO>(c, a, P, s); return true;
     case UPW_UU | UPW_LNRHO: launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_LNRHO>(c, a, P, s); return true;
-    default: return false;
+    default: break;
   }
+  if constexpr (ENT) {  // lupw_ss: entropy runs only
+    switch (c.upw) {
+      case UPW_SS: launch_direct_dt<T, MAG, ENT, UPW_SS>(c, a, P, s); return true;
+      case UPW_UU | UPW_SS: launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_SS>(c, a, P, s); return true;
+      case UPW_LNRHO | UPW_SS: launch_direct_dt<T, MAG, ENT, UPW_LNRHO | UPW_SS>(c, a, P, s); return true;
+      case UPW_UU | UPW_LNRHO | UPW_SS:
+        launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_LNRHO | UPW_SS>(c, a, P, s);
+        return true;
+      default: break;
+    }
+  }
+  return false;
 }
 
 template <typename T>

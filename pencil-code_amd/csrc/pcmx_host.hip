@@ -75,9 +75,11 @@ int check_params(const pcmx_params* p) {
           return fail(PCMX_ERR_ARG, "bcz of variable %d: code %d is not PCMX_BC_S, _A or _A2", v, c);
   }
   if (p->kernel < 0 || p->kernel > 3) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
-  if ((p->lupw_uu != 0 && p->lupw_uu != 1) || (p->lupw_lnrho != 0 && p->lupw_lnrho != 1))
-    return fail(PCMX_ERR_ARG, "lupw_uu and lupw_lnrho are 0 or 1");
-  const bool upw = p->lupw_uu || p->lupw_lnrho;
+  if ((p->lupw_uu != 0 && p->lupw_uu != 1) || (p->lupw_lnrho != 0 && p->lupw_lnrho != 1) ||
+      (p->lupw_ss != 0 && p->lupw_ss != 1))
+    return fail(PCMX_ERR_ARG, "lupw_uu, lupw_lnrho and lupw_ss are 0 or 1");
+  if (p->lupw_ss && !ent) return fail(PCMX_ERR_ARG, "lupw_ss needs the entropy variable");
+  const bool upw = p->lupw_uu || p->lupw_lnrho || p->lupw_ss;
   if (upw && p->kernel == 2) return fail(PCMX_ERR_ARG, "upwinding is not provided by kernel 2");
   if (upw && p->kernel == 3 && (ent || p->lupw_uu != p->lupw_lnrho))
     return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu and lnrho together and without entropy; use kernel 1");
@@ -185,7 +187,7 @@ LaunchCfg cfg_of(const Engine& e, bool dt) {
   c.ent = e.ent;
   c.dt = dt;
   c.zp = e.zplan;
-  c.upw = (e.par.lupw_uu ? UPW_UU : 0) | (e.par.lupw_lnrho ? UPW_LNRHO : 0);
+  c.upw = (e.par.lupw_uu ? UPW_UU : 0) | (e.par.lupw_lnrho ? UPW_LNRHO : 0) | (e.par.lupw_ss ? UPW_SS : 0);
   return c;
 }
 
@@ -475,7 +477,7 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   int kern = par->kernel;
   const int tile = (par->nz >= 4) ? split_tile(par->nx, par->ny) : -1;
   // upwinding of uu alone, of lnrho alone, or with entropy: the direct kernel
-  const bool upw_split = (par->lupw_uu == par->lupw_lnrho) && !(par->lupw_uu && par->iss >= 0);
+  const bool upw_split = (par->lupw_uu == par->lupw_lnrho) && !(par->lupw_uu && par->iss >= 0) && !par->lupw_ss;
   if (kern == 0) kern = (tile >= 0 && upw_split) ? 3 : 1;
   if (kern == 2 && !zmarch_supported(*par)) {
     delete e;

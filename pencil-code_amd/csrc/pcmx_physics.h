@@ -58,15 +58,16 @@ struct Derivs {
   T del2lnrho, del2ss;
   T d2a[3][3];        // D2_j A_i                      (gij_etc d2A(j,j,i))
   T mxa[3][3];
-  // upwind corrections sum_j |u_j| D6_j f of u_x, u_y, u_z, lnrho (doupwind,
-  // sub.f90:7491-7574; D6_j = der6_main with UPWIND, deriv.f90:936-1001),
-  // filled only with upwinding
-  T upw[4];
+  // upwind corrections sum_j |u_j| D6_j f of u_x, u_y, u_z, lnrho, ss
+  // (doupwind, sub.f90:7491-7574; D6_j = der6_main with UPWIND,
+  // deriv.f90:936-1001), filled only with upwinding
+  T upw[5];
 };
 
 // Upwinding masks (compile time): UPW_UU for lupw_uu (hydro.f90:3662),
-// UPW_LNRHO for lupw_lnrho (density.f90:2772-2773).
-constexpr int UPW_UU = 1, UPW_LNRHO = 2;
+// UPW_LNRHO for lupw_lnrho (density.f90:2772-2773), UPW_SS for lupw_ss
+// (entropy.f90:3576).
+constexpr int UPW_UU = 1, UPW_LNRHO = 2, UPW_SS = 4;
 
 // der6_main with UPWIND=.true. along one direction from the symmetric sums
 // s_k = f(+k) + f(-k): fac*(-20 f0 + 15 s1 - 6 s2 + s3) with fac = dx_1/60,
@@ -290,6 +291,7 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
     for (int j = 0; j < 3; ++j) glnTT[j] = (T)P.gamma_m1 * d.glnrho[j] + (T)P.cv1 * d.gss[j];
     del2lnTT = (T)P.gamma_m1 * del2lnrho + (T)P.cv1 * del2ss;
     ugss = u0 * d.gss[0] + u1 * d.gss[1] + u2v * d.gss[2];               // entropy.f90:3563
+    if constexpr ((UPW & UPW_SS) != 0) ugss = ugss - d.upw[4];          // :3576
 #pragma unroll
     for (int j = 0; j < 3; ++j)                                         // entropy.f90:3580
       fpres[j] = -(cs2 * (d.glnrho[j] + glnTT[j]) * (T)P.gamma1);

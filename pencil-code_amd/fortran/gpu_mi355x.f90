@@ -71,6 +71,7 @@ contains
       use EquationOfState, only: rho0
       use Hydro, only: hydro_pushpars2c => pushpars2c
       use Density, only: density_pushpars2c => pushpars2c
+      use Energy, only: energy_pushpars2c => pushpars2c
 !
       real, dimension(:,:,:,:), intent(in) :: f
 !
@@ -78,7 +79,7 @@ contains
       real, pointer :: nu_p, eta_p, chi_p, cs20_p, gamma_p, cp_p
       real, dimension(:), pointer :: gz_p
       real(c_double), dimension(3) :: best
-      integer(KIND=ikind8), dimension(200) :: p_par
+      integer(KIND=ikind8), dimension(500) :: p_par   ! the largest module list (entropy.f90:8672)
       logical, pointer :: lupw
       character(len=512) :: str
       integer :: j, k
@@ -137,6 +138,13 @@ contains
       call density_pushpars2c(p_par)
       call c_f_pointer(transfer(p_par(5), c_null_ptr), lupw)
       par%lupw_lnrho = merge(1, 0, lupw)
+      par%lupw_ss = 0
+      if (lentropy) then                       ! entropy.f90:8704 p_par(29) = lupw_ss
+        p_par = 0
+        call energy_pushpars2c(p_par)
+        call c_f_pointer(transfer(p_par(29), c_null_ptr), lupw)
+        par%lupw_ss = merge(1, 0, lupw)
+      endif
 !
 !  Constant vertical gravity (gravity_simple, gravz_profile='const'): the
 !  profile the Gravity module shares (gravity_simple.f90:255, :516-523).
@@ -256,7 +264,7 @@ contains
 !
       use Forcing, only: forcing_pars_hel, forcing_pushpars2c => pushpars2c
 !
-      integer(KIND=ikind8), dimension(200) :: p_par
+      integer(KIND=ikind8), dimension(500) :: p_par   ! the largest module list (entropy.f90:8672)
       logical, dimension(:), pointer :: lhel
       logical, pointer :: lsecond
       real, pointer :: k1_ff_p

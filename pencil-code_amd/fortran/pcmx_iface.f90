@@ -28,6 +28,7 @@ module pcmx_iface
     real(c_double) :: gravz
     integer(c_int) :: lperi_z
     integer(c_int) :: bcz_bot(8), bcz_top(8)
+    integer(c_int) :: lupw_ss
   end type pcmx_params
 
   ! z boundary codes of pcmx_params (PCMX_BC_* of include/pcmx.h)

@@ -58,7 +58,7 @@ program pcmx_run
   par%cdt = cdt; par%cdtv = cdtv
   par%ldt = merge(1, 0, ldt); par%precision = precision; par%kernel = kernel
   par%lupw_uu = 0; par%lupw_lnrho = 0
-  par%gravz = 0.; par%lperi_z = 1; par%bcz_bot = 0; par%bcz_top = 0
+  par%gravz = 0.; par%lperi_z = 1; par%bcz_bot = 0; par%bcz_top = 0; par%lupw_ss = 0
 
   t = 0d0
   call check(pcmx_register(), 'register')

@@ -51,6 +51,7 @@ class Params(ctypes.Structure):
         ("lupw_uu", ctypes.c_int), ("lupw_lnrho", ctypes.c_int),
         ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
         ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
+        ("lupw_ss", ctypes.c_int),
     ]
 
 
@@ -94,7 +95,8 @@ ENS_NO_GRAPH = 1
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
                 cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
-                kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None):
+                kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None,
+                lupw_ss=False):
     """Parameters of one periodic box with the reference defaults
     (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
     registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
@@ -114,7 +116,7 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, c
     dz = Lz / nz if bcz is None else Lz / (nz - 1)
     return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, dz, nu, eta, chi, cs0, gamma,
                   cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho),
-                  gravz, int(bcz is None), bot, top)
+                  gravz, int(bcz is None), bot, top, int(lupw_ss))
 
 
 _lib = None

@@ -95,6 +95,13 @@ contains
     real, intent(out) :: phase, fact
   end subroutine
 end module Forcing
+module Energy
+  implicit none
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(100) :: p_par
+  end subroutine
+end module Energy
 module File_io
   implicit none
   integer :: parallel_unit=10

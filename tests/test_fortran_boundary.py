@@ -55,7 +55,7 @@ program layout
   p%cdt = 0.7d0; p%cdtv = 0.3d0
   p%ldt = 1; p%precision = 1; p%kernel = 3
   p%lupw_uu = 1; p%lupw_lnrho = 0
-  p%gravz = -0.75d0; p%lperi_z = 0
+  p%gravz = -0.75d0; p%lperi_z = 0; p%lupw_ss = 1
   do k = 1, 8
     p%bcz_bot(k) = k; p%bcz_top(k) = 10 + k
   enddo
@@ -114,7 +114,7 @@ def test_bind_c_layouts_match_the_c_header(tmp_path):
     assert n1 == ctypes.sizeof(pcmx.Params) and n2 == ctypes.sizeof(pcmx.Forcing)
     p = pcmx.Params.from_buffer_copy(pb)
     assert (p.nx, p.ny, p.nz, p.mvar, p.iax, p.kernel, p.lupw_uu, p.lperi_z) == (11, 12, 13, 7, 4, 3, 1, 0)
-    assert (p.dz, p.mu0, p.cdtv, p.gravz) == (0.125, 1.6, 0.3, -0.75)
+    assert (p.dz, p.mu0, p.cdtv, p.gravz, p.lupw_ss) == (0.125, 1.6, 0.3, -0.75, 1)
     assert list(p.bcz_bot) == list(range(1, 9)) and list(p.bcz_top) == list(range(11, 19))
     fr = pcmx.Forcing.from_buffer_copy(fb)
     assert (fr.active, fr.target, fr.scale_sqrt_dt) == (1, 4, 1)

@@ -37,7 +37,7 @@ def oracle_params(gp, dt):
                          chi=gp.chi, cs0=gp.cs0, gamma=gp.gamma, cp=gp.cp, rho0=gp.rho0, mu0=gp.mu0,
                          cdt=gp.cdt, cdtv=gp.cdtv, dt=0.0 if gp.ldt else dt,
                          L=(gp.dx * gp.nx, gp.dy * gp.ny, gp.dz * gp.nz),
-                         lupw_uu=bool(gp.lupw_uu), lupw_lnrho=bool(gp.lupw_lnrho))
+                         lupw_uu=bool(gp.lupw_uu), lupw_lnrho=bool(gp.lupw_lnrho), lupw_ss=bool(gp.lupw_ss))
 
 
 def announce(msg):

@@ -220,3 +220,30 @@ def test_graph_replay_with_upwinding_and_forcing():
     b = _steps(f0, gp, 6, c["dt"], timing=True, calls=2, forcing=kw)
     np.testing.assert_array_equal(a[0], b[0])
     assert a[1] == b[1]
+
+
+@pytest.mark.parametrize("flags", [(False, False, True), (True, True, True), (False, True, True)])
+def test_entropy_upwinding_direct_kernel(flags):
+    """lupw_ss alone and with the other flags (the direct kernel; the split
+    kernel rejects upwinding with entropy), Courant steps."""
+    announce(f"test_entropy_upwinding_direct_kernel[{flags}]: mhdss24")
+    c = {"dt": 0.0}
+    gp = pcmx.make_params(24, lmagnetic=True, lentropy=True, nu=2e-3, eta=2e-3, chi=2e-3, gamma=5 / 3, ldt=True,
+                          cdt=0.4, kernel=0, lupw_uu=flags[0], lupw_lnrho=flags[1], lupw_ss=flags[2])
+    f0 = pcmx_ics.config_state("noise", 24, 7, lmagnetic=True, lentropy=True)
+    g, _, dt = pcmx.advance(f0, gp, 2, 0.0)
+    op = oracle_params(gp, 0.0)
+    o, _, dto, _ = O.advance(f0, op, 2)
+    err = rel_linf(g, o)
+    announce(f"  rel L-inf per variable: {err}")
+    assert np.all(err <= FP64_1STEP * 10), err
+    assert abs(dt - dto) <= 1e-13 * dto
+    op.lupw_ss = 0
+    plain, _, _, _ = O.advance(f0, op, 2)
+    assert rel_linf(o, plain)[4] > 1e3 * FP64_1STEP
+
+
+def test_lupw_ss_needs_entropy():
+    announce("test_lupw_ss_needs_entropy")
+    with pytest.raises(pcmx.PcmxError, match="lupw_ss"):
+        pcmx.initialize_gpu(None, pcmx.make_params(16, lupw_ss=True), nt=0)

@@ -184,3 +184,20 @@ def test_k_dat_reader_and_shell(tmp_path):
     for _ in range(100):
         ik, phase, phi = PF.draw(rng, len(vecs))
         assert 0 <= ik < len(vecs) and -math.pi <= phase < math.pi and 0 <= phi < 2 * math.pi
+
+
+@pytest.mark.parametrize("m", [(1, 2, 3), (0, 4, 1)])
+def test_upwinded_entropy_advection_is_a_damping_per_direction(m):
+    """lupw_ss (entropy.f90:3576): the same damping on ss, nothing else changes."""
+    hs, ks, s = mode(m)
+    A = 0.01
+    f = np.zeros((5,) + s.shape)
+    for j in range(3):
+        f[j] = U0[j]
+    f[4] = A * s
+    kw = dict(L=LS, lentropy=True, gamma=5 / 3, chi=1e-3)
+    off = O.rhs(f, O.make_params(NS, **kw))[0]
+    on = O.rhs(f, O.make_params(NS, lupw_ss=True, **kw))[0]
+    damp = sum(abs(U0[j]) * d6_symbol(ks[j], hs[j]) for j in range(3))
+    np.testing.assert_allclose(on[4] - off[4], damp * A * s, rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(on[:4], off[:4])
