@@ -35,9 +35,10 @@ void launch_direct_dt(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStre
   else launch_direct<T, MAG, ENT, false, UPW>(a, P, s);
 }
 
-// Upwinding: every mask in the direct kernel; both corrections (lupw_uu and
-// lupw_lnrho, the published configuration) without entropy in the split
-// kernel; none in the first z-marching kernel.
+// Upwinding: every mask in the direct kernel; in the split kernel uu and
+// lnrho together without entropy (the published configuration), and with
+// entropy lnrho + ss or uu + lnrho + ss (split_upw_ok); none in the first
+// z-marching kernel.
 template <typename T, bool MAG, bool ENT>
 bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
   if (c.kernel == 3) {
@@ -49,8 +50,14 @@ bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t 
       if (c.upw != (UPW_UU | UPW_LNRHO)) return false;
       if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16, UPW_UU | UPW_LNRHO>(a, P, c.zp, s);
       return launch_split<T, MAG, ENT, ZT32x8, UPW_UU | UPW_LNRHO>(a, P, c.zp, s);
+    } else {
+      // 32x8 tiles only (split_upw_ok)
+      constexpr int M6 = UPW_LNRHO | UPW_SS, M7 = UPW_UU | UPW_LNRHO | UPW_SS;
+      if (c.tile != TILE_32x8) return false;
+      if (c.upw == M6) return launch_split<T, MAG, ENT, ZT32x8, M6>(a, P, c.zp, s);
+      if (c.upw == M7) return launch_split<T, MAG, ENT, ZT32x8, M7>(a, P, c.zp, s);
+      return false;
     }
-    return false;
   }
   if (c.kernel == 2) {
     if (c.upw != 0) return false;

@@ -81,8 +81,10 @@ int check_params(const pcmx_params* p) {
   if (p->lupw_ss && !ent) return fail(PCMX_ERR_ARG, "lupw_ss needs the entropy variable");
   const bool upw = p->lupw_uu || p->lupw_lnrho || p->lupw_ss;
   if (upw && p->kernel == 2) return fail(PCMX_ERR_ARG, "upwinding is not provided by kernel 2");
-  if (upw && p->kernel == 3 && (ent || p->lupw_uu != p->lupw_lnrho))
-    return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu and lnrho together and without entropy; use kernel 1");
+  const int mask = (p->lupw_uu ? UPW_UU : 0) | (p->lupw_lnrho ? UPW_LNRHO : 0) | (p->lupw_ss ? UPW_SS : 0);
+  if (p->kernel == 3 && !split_upw_ok(ent, mask, TILE_32x8))
+    return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu+lnrho (isothermal), lnrho+ss or uu+lnrho+ss (entropy); "
+                "use kernel 1 for other upwinding flags");
   return PCMX_OK;
 }
 
@@ -477,7 +479,8 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   int kern = par->kernel;
   const int tile = (par->nz >= 4) ? split_tile(par->nx, par->ny) : -1;
   // upwinding of uu alone, of lnrho alone, or with entropy: the direct kernel
-  const bool upw_split = (par->lupw_uu == par->lupw_lnrho) && !(par->lupw_uu && par->iss >= 0) && !par->lupw_ss;
+  const int upw_mask = (par->lupw_uu ? UPW_UU : 0) | (par->lupw_lnrho ? UPW_LNRHO : 0) | (par->lupw_ss ? UPW_SS : 0);
+  const bool upw_split = tile >= 0 && split_upw_ok(par->iss >= 0, upw_mask, tile);
   if (kern == 0) kern = (tile >= 0 && upw_split) ? 3 : 1;
   if (kern == 2 && !zmarch_supported(*par)) {
     delete e;
@@ -486,6 +489,11 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   if (kern == 3 && tile < 0) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% 32 == 0 and ny %% 8 == 0, or nx and ny multiples of 16");
+  }
+  if (kern == 3 && !upw_split) {
+    delete e;
+    return fail(PCMX_ERR_ARG, "kernel 3 upwinds with entropy on 32x8 tiles only (nx %% 32 == 0, ny %% 8 == 0); "
+                "use kernel 1");
   }
   e->kernel = kern;
   if (kern == 2) e->zplan = zmarch_plan(*par);
@@ -764,8 +772,10 @@ int pcmx_tune(double* best_ms) {
   if (e.kernel != 3) return fail(PCMX_ERR_ARG, "pcmx_tune tunes the split z-marching kernel (kernel 3) only");
   // candidates: every tile shape the grid allows, chunk lengths dividing nz
   std::vector<std::pair<int, int>> cands;
+  const int upw_mask = cfg_of(e, false).upw;
   for (int tile : {TILE_32x8, TILE_16x16}) {
     if (e.par.nx % tile_tx(tile) || e.par.ny % tile_ty(tile)) continue;
+    if (!split_upw_ok(e.ent, upw_mask, tile)) continue;  // no variant for this tile
     for (int lz : {8, 16, 32, 64, 128, 256, 512, 1024})
       if (lz <= e.par.nz && e.par.nz % lz == 0) cands.push_back({tile, lz});
     if (e.par.nz > 1024 || e.par.nz % 8) cands.push_back({tile, e.par.nz});

@@ -24,7 +24,7 @@ struct LaunchCfg {
   bool mag, ent;
   bool dt;      // kernels 1 and 2: evaluate the Courant terms
   ZPlan zp;     // kernels 2 and 3: the grid of tile columns
-  int upw = 0;  // upwinding mask (UPW_UU | UPW_LNRHO)
+  int upw = 0;  // upwinding mask (UPW_UU, UPW_LNRHO, UPW_SS bits)
 };
 
 struct EnsBox;
@@ -45,6 +45,14 @@ bool launch_ens_f32(const LaunchCfg& c, int stage, const EnsBox* boxes, const En
 
 // Tile of the split kernel for an nx x ny plane: 32 x 8 when it divides the
 // plane, else 16 x 16, else -1 (no split kernel).
+// Upwinding masks the split kernel provides on a tile (the others run on the
+// direct kernel): uu + lnrho without entropy; lnrho + ss or uu + lnrho + ss
+// with entropy on 32x8 tiles.
+inline bool split_upw_ok(bool ent, int mask, int tile) {
+  return mask == 0 || (!ent && mask == (UPW_UU | UPW_LNRHO)) ||
+         (ent && tile == TILE_32x8 && (mask == (UPW_LNRHO | UPW_SS) || mask == (UPW_UU | UPW_LNRHO | UPW_SS)));
+}
+
 inline int split_tile(int nx, int ny) {
   if ((long)nx * ny >= (1L << 31)) return -1;
   if (nx % 32 == 0 && ny % 8 == 0) return TILE_32x8;

@@ -426,7 +426,8 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
   // does variable V take an upwind correction?
   auto upwinded = [](auto v_c) {
     constexpr int V = decltype(v_c)::value;
-    return std::bool_constant<(V < 3 && (UPW & UPW_UU) != 0) || (V == 3 && (UPW & UPW_LNRHO) != 0)>{};
+    return std::bool_constant<(V < 3 && (UPW & UPW_UU) != 0) || (V == 3 && (UPW & UPW_LNRHO) != 0) ||
+                              (ENT && V == ISS && (UPW & UPW_SS) != 0)>{};
   };
 
   // Stencils at plane q of the variables of one staged plane (PT = T2: a

@@ -247,3 +247,53 @@ def test_lupw_ss_needs_entropy():
     announce("test_lupw_ss_needs_entropy")
     with pytest.raises(pcmx.PcmxError, match="lupw_ss"):
         pcmx.initialize_gpu(None, pcmx.make_params(16, lupw_ss=True), nt=0)
+
+
+@pytest.mark.parametrize("lmag", [True, False])
+@pytest.mark.parametrize("flags", [(False, True, True), (True, True, True)])
+def test_entropy_upwinding_split_kernel(flags, lmag):
+    """Entropy with lnrho + ss or uu + lnrho + ss upwinding on the split
+    kernel (32x8 tiles), Courant steps (the deferred first-substep update)."""
+    n = (32, 16, 16)
+    announce(f"test_entropy_upwinding_split_kernel[{flags}, mag={lmag}] n={n}")
+    gp = pcmx.make_params(n, lmagnetic=lmag, lentropy=True, nu=2e-3, eta=2e-3, chi=2e-3, gamma=5 / 3, ldt=True,
+                          cdt=0.4, kernel=3, lupw_uu=flags[0], lupw_lnrho=flags[1], lupw_ss=flags[2],
+                          L=(2 * math.pi, math.pi, math.pi))
+    f0 = pcmx_ics.config_state("shells" if lmag else "noise", n, 7, lmagnetic=lmag, lentropy=True)
+    f0[4] += 0.01 * np.random.default_rng(3).standard_normal(f0[4].shape)
+    g, _, dt = pcmx.advance(f0, gp, 3, 0.0)
+    op = oracle_params(gp, 0.0)
+    o, _, dto, _ = O.advance(f0, op, 3)
+    err = rel_linf(g, o)
+    announce(f"  rel L-inf per variable: {err}")
+    assert np.all(err <= FP64_1STEP * 10), err
+    assert abs(dt - dto) <= 1e-13 * dto
+    op.lupw_uu = op.lupw_lnrho = op.lupw_ss = 0
+    plain, _, _, _ = O.advance(f0, op, 3)
+    assert rel_linf(o, plain)[4] > 1e3 * FP64_1STEP
+
+
+def test_entropy_upwinding_split_kernel_mixed_form():
+    n = (32, 16, 16)
+    announce("test_entropy_upwinding_split_kernel_mixed_form: 60 steps")
+    gp = pcmx.make_params(n, lmagnetic=True, lentropy=True, nu=2e-3, eta=2e-3, chi=2e-3, gamma=5 / 3, ldt=False,
+                          kernel=3, lupw_uu=True, lupw_lnrho=True, lupw_ss=True, precision=pcmx.PREC_MIXED)
+    f0 = pcmx_ics.config_state("shells", n, 7, lmagnetic=True, lentropy=True)
+    f0[4] += 0.01 * np.random.default_rng(3).standard_normal(f0[4].shape)
+    g, _, _ = pcmx.advance(f0, gp, 60, 1e-3)
+    o, _, _, _ = O.advance(f0, oracle_params(gp, 1e-3), 60)
+    err = rel_linf(g, o)
+    announce(f"  mixed rel L-inf per variable: {err}")
+    assert np.all(err <= MIXED_100STEP), err
+
+
+def test_entropy_upwinding_16x16_grid_uses_the_direct_kernel():
+    announce("test_entropy_upwinding_16x16_grid_uses_the_direct_kernel")
+    kw = dict(lmagnetic=True, lentropy=True, gamma=5 / 3, lupw_lnrho=True, lupw_ss=True)
+    with pytest.raises(pcmx.PcmxError, match="32x8"):
+        pcmx.initialize_gpu(None, pcmx.make_params((48, 16, 16), kernel=3, **kw), nt=0)
+    gp = pcmx.make_params((48, 16, 16), kernel=0, **kw)
+    f0 = pcmx_ics.config_state("noise", (48, 16, 16), 2, lmagnetic=True, lentropy=True)
+    g, _, _ = pcmx.advance(f0, gp, 1, 1e-3)
+    o, _, _, _ = O.advance(f0, oracle_params(gp, 1e-3), 1)
+    assert np.all(rel_linf(g, o) <= FP64_1STEP)
