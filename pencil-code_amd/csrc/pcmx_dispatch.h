@@ -1,6 +1,7 @@
 // pcmx_dispatch.h -- template dispatch from a LaunchCfg to the kernel
-// variants; included by the kernel translation units, each of which
-// instantiates it for one precision.
+// variants; included by the kernel translation units (pcmx_kern_phys.hip
+// instantiates launch_phys for one precision and physics, pcmx_kern_f64.hip
+// and pcmx_kern_f32.hip the per-precision entry points and the ensemble).
 #pragma once
 
 #include "pcmx_direct.h"
@@ -85,6 +86,14 @@ bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t 
   }
   return false;
 }
+
+// The instantiations of launch_phys live in pcmx_kern_phys.hip, one unit per
+// (precision, MAG, ENT); the units that only dispatch declare them extern.
+#define PCMX_EXTERN_PHYS(T)                                                                 \
+  extern template bool launch_phys<T, false, false>(const LaunchCfg&, const KArgs&, const Phys&, hipStream_t); \
+  extern template bool launch_phys<T, false, true>(const LaunchCfg&, const KArgs&, const Phys&, hipStream_t);  \
+  extern template bool launch_phys<T, true, false>(const LaunchCfg&, const KArgs&, const Phys&, hipStream_t);  \
+  extern template bool launch_phys<T, true, true>(const LaunchCfg&, const KArgs&, const Phys&, hipStream_t);
 
 template <typename T>
 bool launch_rhs_t(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {

@@ -1,7 +1,11 @@
-// pcmx_kern_f32.hip -- instantiations of every RHS kernel variant in float.
+// pcmx_kern_f32.hip -- the float entry points: the single-box RHS dispatch
+// (its kernel variants are instantiated in pcmx_kern_phys.hip) and the
+// ensemble kernels.
 #include "pcmx_dispatch.h"
 
 namespace pcmx {
+
+PCMX_EXTERN_PHYS(float)
 
 bool launch_rhs_f32(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
   return launch_rhs_t<float>(c, a, P, s);
