@@ -363,6 +363,47 @@ def run_published(n, steps, warmup, tune=True):
     return out
 
 
+CONVSLAB_BC = ["s", "s", "a", "a2", "c1:cT"]
+
+
+def run_convslab(n, steps, warmup):
+    """The physics of samples/conv-slab at nx = ny = n, nz = 2n: entropy
+    hydro in a 1 x 1 x 2 box (xyz0 = -0.5, -0.5, -0.68) with gravz = -1,
+    rotation Omega = 0.1, nu = 4e-3, K-profile conduction (hcond0 = 8e-3,
+    mpoly0 = 1, mpoly1 = 3, mpoly2 = 0, widthss = 0.05, z1 = 0, z2 = 1),
+    top cooling (cool = 15, wcool = 0.2, cs2cool = 0.333333), bcz = s, s, a,
+    a2, c1:cT, lupw_lnrho = lupw_ss = T, Courant steps with cdt = 0.4,
+    cdtv = 0.3, gamma = 1.6666666, cs0 = 0.57735 (samples/conv-slab/start.in,
+    run.in).  Initial state: lnrho = -0.8 (z - z0) and ss = -0.1 (z - z0)
+    plus noise (seed 3), velocity noise; not the reference's piecewise
+    polytrope.  Runs on the direct kernel."""
+    dims = (n, n, 2 * n)
+    interior = pcmx_ics.config_state("noise", dims, 3, lentropy=True)
+    z = -0.68 + 2.0 / (2 * n - 1) * np.arange(2 * n)[:, None, None]
+    interior[:3] *= 0.5
+    interior[3] += -0.8 * (z + 0.68)
+    interior[4] = 0.05 * interior[4] - 0.1 * (z + 0.68)
+    f = pcmx.farray_from_interior(interior)
+    del interior
+    lay = pcmx.layer_defaults(1.6666666, 1.0, -1.0, "c1:cT", hcond0=8e-3, mpoly0=1.0, mpoly1=3.0, mpoly2=0.0,
+                              widthss=0.05, z1=0.0, z2=1.0, cool=15.0, wcool=0.2, cs2cool=0.333333)
+    desc = (f"samples/conv-slab physics {n}x{n}x{2 * n}, 5 FP64 vars: gravz=-1, Omega=0.1, nu=4e-3, K-profile "
+            "(hcond0=8e-3, mpoly 1/3/0), cooling (cool=15, wcool=0.2), bcz s,s,a,a2,c1:cT, "
+            "lupw_lnrho=lupw_ss=T, Courant dt (cdt=0.4, cdtv=0.3); stratified noise state (seed 3)")
+    log(f"bench: conv-slab physics: {desc}")
+    out = {"workload": desc, "unit": "Mcell-updates/s", "grid": list(dims)}
+    for name, prec in (("fp64", pcmx.PREC_FP64), ("mixed", pcmx.PREC_MIXED)):
+        par = pcmx.make_params(dims, lentropy=True, nu=4e-3, gamma=1.6666666, cs0=0.57735, ldt=True, cdt=0.4,
+                               cdtv=0.3, precision=prec, L=(1.0, 1.0, 2.0), gravz=-1.0, bcz=CONVSLAB_BC,
+                               omega=0.1, layers=lay, z0=-0.68, lupw_lnrho=True, lupw_ss=True)
+        r = run_form(f, par, steps, warmup, 0.0, tune=False)
+        out[name] = {"value": round(r["value"], 2), "ms_per_step": round(r["ms_per_step"], 4),
+                     "kernel_ms": round(r["kernel_ms"], 4), "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                     "dt_last": r["dt"], "t": r["t"], "urms": r["urms"]}
+        log(f"bench: conv-slab {name}: {out[name]}")
+    return out
+
+
 ZBOUND_BC = ["s", "s", "a", "a2", "a", "a", "s"]
 
 
@@ -496,6 +537,8 @@ def main():
     ap.add_argument("--published", type=int, default=0, metavar="N",
                     help="time the physics of the reference's published CPU timings (upwinding, helical "
                          "forcing of A, Courant dt) at N^3")
+    ap.add_argument("--convslab", type=int, default=0, metavar="N",
+                    help="time the samples/conv-slab physics at N x N x 2N instead")
     ap.add_argument("--zbound", type=int, default=0, metavar="N",
                     help="time isothermal MHD at N^3 with non-periodic z boundaries and gravity")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -508,6 +551,9 @@ def main():
     threads = min(16, os.cpu_count() or 1)
     if args.io:
         print(json.dumps(run_io(args.io)), flush=True)
+        return
+    if args.convslab:
+        print(json.dumps(run_convslab(args.convslab, args.steps, args.warmup)), flush=True)
         return
     if args.zbound:
         print(json.dumps(run_zbound(args.zbound, args.steps, args.warmup, tune=not args.no_tune)), flush=True)

@@ -68,12 +68,35 @@ typedef struct pcmx_params {
   int bcz_bot[8], bcz_top[8];/* run_pars bcz per variable slot when lperi_z == 0: PCMX_BC_* */
   int lupw_ss;               /* entropy_run_pars lupw_ss: upwinded u.grad ss (entropy.f90:3576);
                                 entropy runs, direct kernel */
+  /* Layer physics of samples/conv-slab, on the direct kernel (kernel 1; kernel 0
+   * selects it when any of these is on).  All zero: off. */
+  double omega, theta;       /* hydro_run_pars Omega, theta (degrees): Coriolis force
+                                (coriolis_cartesian, hydro.f90:6339-6413) */
+  int lheatc_kprof;          /* entropy_run_pars iheatcond='K-profile' (lmultilayer, lgravz):
+                                ds/dt += K(z)/rho (del2 lnT + grad lnT.(grad lnT + grad ln K)) */
+  int cooling_profile;       /* 0 'gaussian' about the top, 1 'step' at z2 (entropy.f90:1319-1368) */
+  double hcond0, hcond1, hcond2, widthss; /* K(z) = hcond0 (1 + (hcond1-1) step(z,z1,-widthss)
+                                             + (hcond2-1) step(z,z2,widthss)) (entropy.f90:7741-7769) */
+  double z1, z2;             /* grav_run_pars z1, z2 */
+  double luminosity, wheat;  /* bottom heating, a half-Gaussian of width wheat (:1374-1386) */
+  double cool, wcool, cs2cool; /* top cooling towards cs2cool (get_heat_cool_gravz :6761-6828);
+                                  luminosity or cool needs cs2cool > 0 */
+  double fbot, ftop;         /* heat fluxes of PCMX_BC_C1 (initialize_energy :845-890) */
+  double cs2bot, cs2top;     /* sound speeds squared of PCMX_BC_CT */
+  double z0;                 /* init_pars xyz0(3): z of the first grid point (non-periodic z) or of
+                                the lower box face (periodic z) */
+  double lxyz[3];            /* init_pars Lxyz (heating normalisation, top of the box) */
 } pcmx_params;
 
 /* z boundary conditions of bc_sym_z (boundcond.f90:3842-3900) */
 #define PCMX_BC_S 1  /* 's':  f(n1-i) = f(n1+i) */
 #define PCMX_BC_A 2  /* 'a':  f(n1-i) = -f(n1+i), f(n1) = 0 */
 #define PCMX_BC_A2 3 /* 'a2': f(n1-i) = 2 f(n1) - f(n1+i) */
+/* entropy-only rules (slot iss), with lnrho's rule applied first */
+#define PCMX_BC_C1 4 /* 'c1': constant heat flux fbot / ftop through K(z) at the boundary
+                        (bc_ss_flux, entropy_bcs.f90:93-235; needs lheatc_kprof) */
+#define PCMX_BC_CT 5 /* 'cT': constant temperature, cs2 = cs2bot / cs2top at the boundary
+                        (bc_ss_temp_z, entropy_bcs.f90:1329-1457) */
 
 /* Last error message (thread-local to the calling thread). */
 const char* pcmx_last_error(void);

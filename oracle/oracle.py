@@ -35,10 +35,47 @@ class Params(ctypes.Structure):
         ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
         ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
         ("lupw_ss", ctypes.c_int),
+        ("omega", ctypes.c_double), ("theta", ctypes.c_double),
+        ("lheatc_kprof", ctypes.c_int), ("cooling_profile", ctypes.c_int),
+        ("hcond0", ctypes.c_double), ("hcond1", ctypes.c_double), ("hcond2", ctypes.c_double),
+        ("widthss", ctypes.c_double), ("z1", ctypes.c_double), ("z2", ctypes.c_double),
+        ("luminosity", ctypes.c_double), ("wheat", ctypes.c_double),
+        ("cool", ctypes.c_double), ("wcool", ctypes.c_double), ("cs2cool", ctypes.c_double),
+        ("fbot", ctypes.c_double), ("ftop", ctypes.c_double),
+        ("cs2bot", ctypes.c_double), ("cs2top", ctypes.c_double),
+        ("z0", ctypes.c_double),
     ]
 
 
-BC_CODES = {"s": 1, "a": 2, "a2": 3}
+BC_CODES = {"s": 1, "a": 2, "a2": 3, "c1": 4, "cT": 5}
+
+# conv-slab layer fields of Params (set through make_params(layers=...))
+LAYER_FIELDS = ("lheatc_kprof", "cooling_profile", "hcond0", "hcond1", "hcond2", "widthss", "z1", "z2",
+                "luminosity", "wheat", "cool", "wcool", "cs2cool", "fbot", "ftop", "cs2bot", "cs2top")
+
+
+def layer_defaults(gamma, cp, gravz, bcz_ss, hcond0=0.0, mpoly0=1.5, mpoly1=1.5, mpoly2=1.5, widthss=0.05,
+                   z1=0.0, z2=1.0, luminosity=0.0, wheat=0.1, cool=0.0, wcool=0.1, cs2cool=0.0,
+                   cooling_profile="gaussian", cs2top=None, cs2bot=None, fbot=None, ftop=None,
+                   kprof=True):
+    """The conv-slab layer fields as initialize_energy derives them
+    (entropy.f90:805-890): hcond1 = (mpoly1+1)/(mpoly0+1), hcond2 likewise;
+    Fbot (Ftop) = -gamma/(cp*gamma_m1)*hcond0*gravz/(mpoly0+1) when the bottom
+    (top) condition of ss is 'c1', else 0; with cooling, cs2top =
+    (cool*cs2cool + cool2*cs2cool2)/(cool+cool2), cool2 = cs2cool2 = 0.
+    bcz_ss is the run.in entry of ss, e.g. 'c1:cT'."""
+    lo, _, hi = bcz_ss.partition(":")
+    hi = hi or lo
+    gamma_m1 = gamma - 1.0
+    flux = -gamma / (cp * gamma_m1) * hcond0 * gravz / (mpoly0 + 1)
+    if cs2top is None:
+        cs2top = (cool * cs2cool + 0.0 * 0.0) / (cool + 0.0) if cool != 0.0 else 0.0
+    return dict(lheatc_kprof=int(kprof), cooling_profile={"gaussian": 0, "step": 1}[cooling_profile],
+                hcond0=hcond0, hcond1=(mpoly1 + 1.) / (mpoly0 + 1.), hcond2=(mpoly2 + 1.) / (mpoly0 + 1.),
+                widthss=widthss, z1=z1, z2=z2, luminosity=luminosity, wheat=wheat, cool=cool, wcool=wcool,
+                cs2cool=cs2cool, fbot=(flux if lo == "c1" else 0.0) if fbot is None else fbot,
+                ftop=(flux if hi == "c1" else 0.0) if ftop is None else ftop,
+                cs2bot=0.0 if cs2bot is None else cs2bot, cs2top=cs2top)
 
 
 def bc_codes(bcz, nv):
@@ -69,17 +106,27 @@ def make_forcing(kk, phase, fact, coef1, coef2, fda=(1.0, 1.0, 1.0), k1_ff=1.0, 
 
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0,
                 cs0=1.0, gamma=1.0, cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25,
-                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None, lupw_ss=False):
+                dt=1e-3, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None, lupw_ss=False,
+                omega=0.0, theta=0.0, layers=None, z0=None):
     """Run parameters with the reference defaults (cdata.f90:235-239,
     eos_idealgas.f90:40-46).  bcz: None (periodic z) or one run.in entry per
-    variable ('s', 'a', 'a2', or 'bottom:top')."""
+    variable ('s', 'a', 'a2', 'c1', 'cT', or 'bottom:top').  layers: the
+    conv-slab fields (layer_defaults).  z0: z of the first grid point
+    (default -Lz/2)."""
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
     Lx, Ly, Lz = (L, L, L) if np.isscalar(L) else L
     nv = 4 + int(lentropy) + 3 * int(lmagnetic)
     bot, top = bc_codes(bcz, nv) if bcz is not None else ((ctypes.c_int * 8)(), (ctypes.c_int * 8)())
-    return Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
-                  cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho),
-                  gravz, int(bcz is None), bot, top, int(lupw_ss))
+    p = Params(nx, ny, nz, int(lmagnetic), int(lentropy), Lx, Ly, Lz, nu, eta, chi,
+               cs0, gamma, cp, rho0, mu0, cdt, cdtv, dt, int(lupw_uu), int(lupw_lnrho),
+               gravz, int(bcz is None), bot, top, int(lupw_ss))
+    p.omega, p.theta = omega, theta
+    p.z0 = -Lz / 2 if z0 is None else z0
+    for k, v in (layers or {}).items():
+        if k not in LAYER_FIELDS:
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
 
 
 def build():
@@ -109,6 +156,8 @@ def lib():
         L.pco_advance.argtypes = [dp, dp, pp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
         L.pco_addforce.argtypes = [dp, pp, ctypes.POINTER(Forcing)]
         L.pco_boundconds.argtypes = [dp, pp]
+        L.pco_layer_profile.argtypes = [pp, ctypes.c_int, dp]
+        L.pco_layer_profile.restype = None
         for name in ("pco_der", "pco_der2", "pco_derij", "pco_fill_ghosts", "pco_pde",
                      "pco_advance", "pco_eos_constants", "pco_addforce", "pco_boundconds"):
             getattr(L, name).restype = None
@@ -218,3 +267,11 @@ def boundconds(f, p):
     ghost zones (nvar, mz, my, mx): periodic x and y, periodic or bcz in z."""
     lib().pco_boundconds(_ptr(f), ctypes.byref(p))
     return f
+
+
+def layer_profile(p):
+    """(nz, 4) array of {hcond, dlnhcond/dz, profz_heat, profz_cool} per z plane."""
+    out = np.zeros((p.nz, 4))
+    for n in range(p.nz):
+        lib().pco_layer_profile(ctypes.byref(p), n, _ptr(out[n]))
+    return out

@@ -117,16 +117,80 @@ static void bc_sym_z(double* a, int mx, int my, int nz, int code, int top) {
   if (code == PCO_BC_A) memset(a + (long)nb * pl, 0, sizeof(double) * pl); /* set bdry value=0 */
 }
 
+/* 'c1' and 'cT' on ss (log density, no reference state), after the ghost
+ * zones of lnrho are set (lnrho precedes ss in the variable loop of
+ * boundconds_z).  Plane nb is n1 (bottom) or n2 (top); dz2[i] is
+ * dz2_bound(-i) = 2*(z(n1+i)-z(n1)) or dz2_bound(i) = 2*(z(n2)-z(n2-i))
+ * (grid.f90:3149-3151).
+ * 'c1' (bc_ss_flux, entropy_bcs.f90:93-235, the K-profile branch):
+ *   cs2 = cs20*exp(gamma_m1*(lnrho(nb)-lnrho0)+cv1*ss(nb)), tmp = cp*FbotKbot/cs2,
+ *   bottom ss(n1-i) = ss(n1+i) + (cp-cv)*(dlnrho + dz2_bound(-i)*tmp),
+ *          dlnrho = lnrho(n1+i) - lnrho(n1-i) (getdlnrho_z, density_methods.f90:409-426);
+ *   top    ss(n2+i) = ss(n2-i) + (cp-cv)*(-dlnrho - dz2_bound(i)*tmp),
+ *          dlnrho = lnrho(n2+i) - lnrho(n2-i), tmp with FtopKtop.
+ * 'cT' (bc_ss_temp_z, entropy_bcs.f90:1329-1457): tmp = 2*cv*log(cs2b/cs20),
+ *   ss(nb) = 0.5*tmp - (cp-cv)*(lnrho(nb)-lnrho0),
+ *   ss(nb-+i) = -ss(nb+-i) + tmp - (cp-cv)*(lnrho(nb+-i)+lnrho(nb-+i)-2*lnrho0). */
+static void bc_ss_z(double* ss, const double* lr, const pco_params* p, int code, int top) {
+  const int mx = p->nx + 2 * NG, my = p->ny + 2 * NG;
+  const long pl = (long)mx * my;
+  const int nb = top ? NG + p->nz - 1 : NG;
+  double e[8];
+  pco_eos_constants(p, e);
+  const double cs20 = e[0], gamma_m1 = e[1], cv = e[4], cv1 = e[5], lnrho0 = e[6], cp = p->cp;
+  const double dz = p->Lz / (p->nz - 1);
+  double zc[2 * NG + 1]; /* z(nb + j), j = -NG..NG */
+  for (int j = -NG; j <= NG; ++j) zc[j + NG] = p->z0 + (nb - NG + j) * dz;
+  double* sb = ss + (long)nb * pl;
+  const double* lb = lr + (long)nb * pl;
+  if (code == PCO_BC_C1) {
+    const double hk = top ? p->hcond0 * p->hcond2 : p->hcond0 * p->hcond1;
+    const double fk = hk != 0.0 ? (top ? p->ftop : p->fbot) / hk : 0.0; /* FbotKbot / FtopKtop (:851-873) */
+    for (long q = 0; q < pl; ++q) {
+      const double cs2 = cs20 * exp(gamma_m1 * (lb[q] - lnrho0) + cv1 * sb[q]);
+      const double tmp = cp * fk / cs2;
+      for (int i = 1; i <= NG; ++i) {
+        const long in = q + (long)(top ? -i : i) * pl, gh = q + (long)(top ? i : -i) * pl;
+        if (!top) {
+          const double dz2 = 2. * (zc[NG + i] - zc[NG]);
+          const double dlnrho = lb[in] - lb[gh];
+          sb[gh] = sb[in] + (cp - cv) * (dlnrho + dz2 * tmp);
+        } else {
+          const double dz2 = 2. * (zc[NG] - zc[NG - i]);
+          const double dlnrho = lb[gh] - lb[in];
+          sb[gh] = sb[in] + (cp - cv) * (-dlnrho - dz2 * tmp);
+        }
+      }
+    }
+  } else if (code == PCO_BC_CT) {
+    const double cs2b = top ? p->cs2top : p->cs2bot;
+    const double tmp = 2 * cv * log(cs2b / cs20);
+    for (long q = 0; q < pl; ++q) {
+      sb[q] = 0.5 * tmp - (cp - cv) * (lb[q] - lnrho0);
+      for (int i = 1; i <= NG; ++i) {
+        const long in = q + (long)(top ? -i : i) * pl, gh = q + (long)(top ? i : -i) * pl;
+        sb[gh] = -sb[in] + tmp - (cp - cv) * (lb[in] + lb[gh] - 2 * lnrho0);
+      }
+    }
+  }
+}
+
 void pco_boundconds(double* f, const pco_params* p) {
   const int nvar = pco_nvar(p);
   pco_fill_ghosts(f, p->nx, p->ny, p->nz, nvar);
   if (p->lperi_z) return;
   const int mx = p->nx + 2 * NG, my = p->ny + 2 * NG;
   const long mw = (long)mx * my * (p->nz + 2 * NG);
-  for (int k = 0; k < nvar; ++k) {
-    bc_sym_z(f + k * mw, mx, my, p->nz, p->bcz_bot[k], 0);
-    bc_sym_z(f + k * mw, mx, my, p->nz, p->bcz_top[k], 1);
-  }
+  const int iss = pco_iss(p);
+  for (int k = 0; k < nvar; ++k)
+    for (int top = 0; top < 2; ++top) {
+      const int code = top ? p->bcz_top[k] : p->bcz_bot[k];
+      if (code == PCO_BC_C1 || code == PCO_BC_CT) {
+        if (k == iss) bc_ss_z(f + k * mw, f + 3 * mw, p, code, top);
+      } else {
+        bc_sym_z(f + k * mw, mx, my, p->nz, code, top);
+      }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -564,16 +628,95 @@ static void calc_pencils(const ctx* c, const double* f, int m, int n, pencils* p
 }
 
 /* ------------------------------------------------------------------ */
+/* pi (cparam.f90), step and der_step (sub.f90:4975-5015), tini = 5*tiny
+ * (cparam.f90:129). */
+static const double PCO_PI = 3.14159265358979323846264338327950;
+static const double TINI = 5 * 2.2250738585072014e-308;
+static double step_fn(double x, double x0, double w) { return 0.5 * (1 + tanh((x - x0) / (w + TINI))); }
+static double der_step_fn(double x, double x0, double w) {
+  const double arg = fabs((x - x0) / (w + TINI));
+  if (fabs(arg) >= 8.) return 2. / w * exp(-2. * fabs(arg));
+  const double c = cosh(arg);
+  return 0.5 / (w * (c * c));
+}
+
+/* z profiles at interior plane n: hcond and dlnhcond of get_gravz_heatcond
+ * (entropy.f90:7741-7769), profz_heat (:1374-1386: luminosity times a
+ * half-Gaussian at the bottom normalised by the horizontal area), and
+ * profz_cool (:1319-1368: 'gaussian' about the top, or 'step' at z2, times
+ * cool).  z(n) = xyz0(3) + n*dz with dz = Lz/(nz-1) (non-periodic z). */
+void pco_layer_profile(const pco_params* p, int n, double out[4]) {
+  const double dz = p->lperi_z ? p->Lz / p->nz : p->Lz / (p->nz - 1);
+  const double z = p->z0 + (p->lperi_z ? n + 0.5 : (double)n) * dz;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (p->lheatc_kprof) {
+    double hc = 1. + (p->hcond1 - 1.) * step_fn(z, p->z1, -p->widthss) + (p->hcond2 - 1.) * step_fn(z, p->z2, p->widthss);
+    const double dl = ((p->hcond1 - 1.) * der_step_fn(z, p->z1, -p->widthss) +
+                       (p->hcond2 - 1.) * der_step_fn(z, p->z2, p->widthss)) / hc;
+    out[0] = p->hcond0 * hc;
+    out[1] = dl;
+  }
+  if (p->luminosity != 0.0) {
+    const double t = (z - p->z0) / p->wheat;
+    double h = p->luminosity * exp(-0.5 * (t * t)) / (sqrt(PCO_PI / 2.) * p->wheat);
+    if (p->ny == 1 && p->nx > 1) h = h / p->Lx;
+    else if (p->nx == 1 && p->ny > 1) h = h / p->Ly;
+    else if (p->nx > 1 && p->ny > 1) h = h / (p->Lx * p->Ly);
+    out[2] = h;
+  }
+  if (p->cool != 0.0) {
+    double c;
+    if (p->cooling_profile == 1) {
+      c = step_fn(z, p->z2, p->wcool);
+    } else {
+      const double t = ((p->z0 + p->Lz) - z) / p->wcool;
+      c = exp(-0.5 * (t * t));
+    }
+    out[3] = p->cool * c;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* calc_all_rhs, equ.f90:1223-1366: duu_dt, dlnrho_dt, denergy_dt, daa_dt */
 static void calc_rhs(const ctx* c, double* df, int m, int n, pencils* pc) {
   const geom* g = &c->g;
   const int nx = g->nx;
   const pco_params* p = c->p;
 #define DF(k) (&F(g, df, NG, m, n, (k)))
-  /* duu_dt: advec_uu -ugu (hydro.f90:4384), calc_viscous_force +fvisc (viscosity.f90:2823) */
+  /* duu_dt: advec_uu -ugu (hydro.f90:4384), Coriolis force (:4494),
+   * calc_viscous_force +fvisc (:4518, viscosity.f90:2823) */
   for (int j = 0; j < 3; ++j) {
     double* d = DF(j);
     for (int l = 0; l < nx; ++l) d[l] = d[l] - PC(pc, P_UGU + j)[l];
+  }
+  if (p->omega != 0.0) {
+    /* coriolis_cartesian (hydro.f90:6339-6413): theta = 0: c2 = 2*Omega,
+     * du_x += c2*u_y, du_y -= c2*u_x; else Omega = (-sin theta, 0, cos theta),
+     * c2 = 2*Omega*cos(theta*dtor), s2 = -2*Omega*sin(theta*dtor). */
+    double* dux = DF(0);
+    double* duy = DF(1);
+    double* duz = DF(2);
+    const double* ux = PC(pc, P_U);
+    const double* uy = PC(pc, P_U + 1);
+    const double* uz = PC(pc, P_U + 2);
+    if (p->theta == 0.0) {
+      const double c2 = 2 * p->omega;
+      for (int l = 0; l < nx; ++l) {
+        dux[l] = dux[l] + c2 * uy[l];
+        duy[l] = duy[l] - c2 * ux[l];
+      }
+    } else {
+      const double dtor = PCO_PI / 180.;
+      const double c2 = 2 * p->omega * cos(p->theta * dtor), s2 = -2 * p->omega * sin(p->theta * dtor);
+      for (int l = 0; l < nx; ++l) {
+        dux[l] = dux[l] + c2 * uy[l];
+        duy[l] = duy[l] - c2 * ux[l] + s2 * uz[l];
+        duz[l] = duz[l] - s2 * uy[l];
+      }
+    }
+  }
+  for (int j = 0; j < 3; ++j) {
+    double* d = DF(j);
     for (int l = 0; l < nx; ++l) d[l] = d[l] + PC(pc, P_FVISC + j)[l];
   }
   /* dlnrho_dt -> continuity_eq: density_rhs = -divu - uglnrho (density.f90:2929-2930, :3059) */
@@ -608,6 +751,20 @@ static void calc_rhs(const ctx* c, double* df, int m, int n, pencils* pc) {
     /* -ugss (:3758), viscous heat TT1*visc_heat (viscosity.f90:2764) */
     for (int l = 0; l < nx; ++l) ds[l] = ds[l] - PC(pc, P_UGSS)[l];
     for (int l = 0; l < nx; ++l) ds[l] = ds[l] + PC(pc, P_TT1)[l] * PC(pc, P_VISCHEAT)[l];
+    double zl[4];
+    pco_layer_profile(p, n - NG, zl);
+    /* calc_heatcond (entropy.f90:3800, :5913-6151) with iheatcond='K-profile':
+     * glnThcond = glnTT + glhc, glhc = (0, 0, dlnhcond(z));
+     * thdiff = rho1*hcond*(del2lnTT + glnTT.glnThcond); chix = rho1*hcond*cv1 */
+    if (p->lheatc_kprof)
+      for (int l = 0; l < nx; ++l) {
+        const double g1 = PC(pc, P_GLNTT)[l], g2v = PC(pc, P_GLNTT + 1)[l], g3 = PC(pc, P_GLNTT + 2)[l];
+        double g2 = g1 * (g1 + 0.0);
+        g2 = g2 + g2v * (g2v + 0.0);
+        g2 = g2 + g3 * (g3 + zl[1]);
+        const double thdiff = PC(pc, P_RHO1)[l] * zl[0] * (PC(pc, P_DEL2LNTT)[l] + g2);
+        ds[l] = ds[l] + thdiff;
+      }
     /* calc_heatcond_constchi (entropy.f90:4852-4947):
      * g2 = (glnrho+glnTT).glnTT ; thdiff = chi*cp*(del2lnTT+g2) */
     for (int l = 0; l < nx; ++l) {
@@ -620,10 +777,22 @@ static void calc_rhs(const ctx* c, double* df, int m, int n, pencils* pc) {
       double thdiff = p->chi * p->cp * (PC(pc, P_DEL2LNTT)[l] + g2);
       ds[l] = ds[l] + thdiff;
     }
+    /* calc_heat_cool (:3839, :6387-6560) through get_heat_cool_gravz
+     * (:6761-6828) when luminosity or cool is set: heat = profz_heat -
+     * profz_cool*(cs2-cs2cool)/cs2cool (lcooling_to_cs2cool); df(ss) += TT1*rho1*heat */
+    if (p->luminosity != 0.0 || p->cool != 0.0)
+      for (int l = 0; l < nx; ++l) {
+        double heat = 0.0;
+        if (p->luminosity != 0.0) heat = zl[2];
+        heat = heat - zl[3] * (PC(pc, P_CS2)[l] - p->cs2cool) / p->cs2cool;
+        ds[l] = ds[l] + PC(pc, P_TT1)[l] * PC(pc, P_RHO1)[l] * heat;
+      }
     if (c->lupdate_courant_dt) {
       double* md = PC(pc, P_MAXDIFFUS);
       for (int l = 0; l < nx; ++l) {
-        double d = 0.0 + (p->gamma * p->chi) * g->dxyz_2; /* diffus_chi */
+        double d = 0.0; /* diffus_chi */
+        if (p->lheatc_kprof) d = d + PC(pc, P_RHO1)[l] * zl[0] * c->cv1 * g->dxyz_2; /* chix*dxyz_2 (:6094) */
+        d = d + (p->gamma * p->chi) * g->dxyz_2;                                      /* (:4907) */
         md[l] = md[l] > d ? md[l] : d;
       }
     }

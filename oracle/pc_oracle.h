@@ -51,11 +51,31 @@ typedef struct {
   int lperi_z;             /* init_pars lperi(3); 0: the bcz rules below hold in z */
   int bcz_bot[8], bcz_top[8]; /* per variable: PCO_BC_S, PCO_BC_A, PCO_BC_A2 (run_pars bcz) */
   int lupw_ss;             /* entropy_run_pars lupw_ss: upwinded u.grad ss (entropy.f90:3576) */
+  /* samples/conv-slab physics (lgravz):                                      */
+  double omega, theta;     /* hydro_run_pars Omega, theta (degrees): Coriolis force
+                              (coriolis_cartesian, hydro.f90:6339-6413) */
+  int lheatc_kprof;        /* entropy_run_pars iheatcond='K-profile' with lmultilayer */
+  int cooling_profile;     /* 0 'gaussian', 1 'step' (entropy.f90:1319-1368) */
+  double hcond0, hcond1, hcond2, widthss; /* K(z) = hcond0*(1 + (hcond1-1) step(z,z1,-widthss)
+                                             + (hcond2-1) step(z,z2,widthss)) */
+  double z1, z2;           /* grav_run_pars z1, z2 */
+  double luminosity, wheat;/* bottom heating (entropy.f90:1374-1386) */
+  double cool, wcool, cs2cool; /* top cooling towards cs2cool (get_heat_cool_gravz) */
+  double fbot, ftop;       /* heat fluxes of 'c1' (initialize_energy, entropy.f90:845-890) */
+  double cs2bot, cs2top;   /* sound speed squared of 'cT' */
+  double z0;               /* init_pars xyz0(3): z of the first grid point (non-periodic z) */
 } pco_params;
 
 #define PCO_BC_S 1   /* 's':  symmetric about the boundary point */
 #define PCO_BC_A 2   /* 'a':  antisymmetric, boundary value 0 */
 #define PCO_BC_A2 3  /* 'a2': antisymmetric about the boundary value */
+#define PCO_BC_C1 4  /* 'c1' on ss: constant heat flux (bc_ss_flux, entropy_bcs.f90:93-235) */
+#define PCO_BC_CT 5  /* 'cT' on ss: constant temperature (bc_ss_temp_z, entropy_bcs.f90:1329-1457) */
+
+/* z profiles of the conv-slab physics at interior plane n (0-based):
+ * out = {hcond, dlnhcond/dz, profz_heat, profz_cool} (get_gravz_heatcond,
+ * entropy.f90:7741-7769; heating and cooling profiles :1319-1386). */
+void pco_layer_profile(const pco_params* p, int n, double out[4]);
 
 /* Helical forcing of one step, as forcing_pars_hel returns it
  * (forcing.f90:1707-2001) plus the grid origin and the forced variable. */

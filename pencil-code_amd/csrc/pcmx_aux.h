@@ -89,9 +89,29 @@ __global__ void __launch_bounds__(256) k_pack_ghosts(const double* __restrict__ 
 // planes of 'a' variables, as boundconds_z does to the f-array before the
 // RHS.  Bottom: 's' f(-i) = f(i), 'a' -f(i), 'a2' 2*f(0) - f(i); top (n2 =
 // nz-1): 's' f(n2-i), 'a' -f(n2-i), 'a2' f(n2) + (f(n2) - f(n2-i)).
+// The entropy slot may carry 'c1' or 'cT' (log density, no reference
+// state), which read lnrho (slot 3) with its own rule applied first, as the
+// variable loop of boundconds_z does:
+//  'c1' (bc_ss_flux, entropy_bcs.f90:93-235, the K-profile branch):
+//    cs2 = cs20*exp(gamma_m1*(lnrho(nb)-lnrho0)+cv1*ss(nb)), tmp = cp*FK/cs2,
+//    bottom ss(-i) = ss(i) + (cp-cv)*((lnrho(i)-lnrho(-i)) + dz2_bound(-i)*tmp),
+//    top ss(n2+i) = ss(n2-i) + (cp-cv)*(-(lnrho(n2+i)-lnrho(n2-i)) - dz2_bound(i)*tmp);
+//  'cT' (bc_ss_temp_z, entropy_bcs.f90:1329-1457), tmpT = 2*cv*log(cs2b/cs20):
+//    ss(nb) = 0.5*tmpT - (cp-cv)*(lnrho(nb)-lnrho0) (with zero_a),
+//    ghost = -ss(mirror) + tmpT - (cp-cv)*(lnrho(mirror)+lnrho(ghost)-2*lnrho0).
 struct ZBC {
   int bot[8], top[8];
+  int iss;                            // entropy slot, or -1
+  double cs20, gamma_m1, cv1, lnrho0, cp, cpmcv;
+  double fk[2];                       // FbotKbot, FtopKtop
+  double tmpT[2];                     // 2*cv*log(cs2bot/cs20), 2*cv*log(cs2top/cs20)
+  double dz2[2][3];                   // dz2_bound(-i) (bottom), dz2_bound(i) (top), i = 1..3
 };
+
+__device__ __forceinline__ double sym_ghost(int c, double fb, double fi, bool top) {
+  return c == PCMX_BC_S ? fi : (c == PCMX_BC_A ? -fi : (top ? fb + (fb - fi) : 2 * fb - fi));
+}
+
 __global__ void __launch_bounds__(256) k_zbc(double* __restrict__ f, double* __restrict__ gh, ZBC bc, int nvar,
                                              long nxy, int nz, int zero_a) {
   const long q = blockIdx.x * 256L + threadIdx.x;
@@ -101,12 +121,45 @@ __global__ void __launch_bounds__(256) k_zbc(double* __restrict__ f, double* __r
   double* gv = gh + (long)v * 6 * nxy;
   const int n2 = nz - 1;
   const double fb = fv[q], ft = fv[(long)n2 * nxy + q];
+  const double* lr = f + 3L * nxy * nz;
 #pragma unroll
-  for (int i = 1; i <= 3; ++i) {
-    const double ib = fv[(long)i * nxy + q], it = fv[(long)(n2 - i) * nxy + q];
-    const int cb = bc.bot[v], ct = bc.top[v];
-    gv[(long)(3 - i) * nxy + q] = cb == PCMX_BC_S ? ib : (cb == PCMX_BC_A ? -ib : 2 * fb - ib);
-    gv[(long)(2 + i) * nxy + q] = ct == PCMX_BC_S ? it : (ct == PCMX_BC_A ? -it : ft + (ft - it));
+  for (int side = 0; side < 2; ++side) {
+    const int c = side ? bc.top[v] : bc.bot[v];
+    const long nb = side ? n2 : 0;
+    if (c <= PCMX_BC_A2) {
+#pragma unroll
+      for (int i = 1; i <= 3; ++i) {
+        const double fi = fv[(nb + (side ? -i : i)) * nxy + q];
+        gv[(long)(side ? 2 + i : 3 - i) * nxy + q] = sym_ghost(c, side ? ft : fb, fi, side != 0);
+      }
+      continue;
+    }
+    // entropy rules: lnrho's boundary value (0 under zero_a when its rule is
+    // 'a', set before ss in the reference) and its ghosts
+    const int lc = side ? bc.top[3] : bc.bot[3];
+    const double lb = (zero_a && lc == PCMX_BC_A) ? 0.0 : lr[nb * nxy + q];
+    const double sb = side ? ft : fb;
+    if (c == PCMX_BC_C1) {
+      const double cs2 = bc.cs20 * exp(bc.gamma_m1 * (lb - bc.lnrho0) + bc.cv1 * sb);
+      const double tmp = bc.cp * bc.fk[side] / cs2;
+#pragma unroll
+      for (int i = 1; i <= 3; ++i) {
+        const long in = (nb + (side ? -i : i)) * nxy + q;
+        const double li = lr[in], lg = sym_ghost(lc, lb, li, side != 0);
+        const double g = side ? fv[in] + bc.cpmcv * (-(lg - li) - bc.dz2[1][i - 1] * tmp)
+                              : fv[in] + bc.cpmcv * ((li - lg) + bc.dz2[0][i - 1] * tmp);
+        gv[(long)(side ? 2 + i : 3 - i) * nxy + q] = g;
+      }
+    } else {  // PCMX_BC_CT
+      const double t = bc.tmpT[side];
+      if (zero_a) fv[nb * nxy + q] = 0.5 * t - bc.cpmcv * (lb - bc.lnrho0);
+#pragma unroll
+      for (int i = 1; i <= 3; ++i) {
+        const long in = (nb + (side ? -i : i)) * nxy + q;
+        const double li = lr[in], lg = sym_ghost(lc, lb, li, side != 0);
+        gv[(long)(side ? 2 + i : 3 - i) * nxy + q] = -fv[in] + t - bc.cpmcv * (li + lg - 2 * bc.lnrho0);
+      }
+    }
   }
   if (zero_a) {
     if (bc.bot[v] == PCMX_BC_A) fv[q] = 0.0;

@@ -20,7 +20,9 @@ struct Slots {
   static constexpr int IAX = MAG ? (ENT ? 5 : 4) : -1;
 };
 
-template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0>
+// LAY: the samples/conv-slab layer physics (Coriolis force, K-profile
+// conduction, heating and cooling; Phys::zlay row of this z plane).
+template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0, bool LAY = false>
 __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
   using S = Slots<ENT, MAG>;
   constexpr int NV = S::NV;
@@ -149,7 +151,8 @@ __global__ void __launch_bounds__(256) k_rhs_direct(KArgs a, Phys P) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) w[v] = a.first ? 0.0 : a.alpha * a.w[v * nxyz + idx];
     double dt1 = 0.0, u2 = 0.0, b2 = 0.0;
-    rhs_point<T, MAG, ENT, DT, UPW>(d, P, w, S::ISS, S::IAX, &dt1, &u2, &b2);
+    rhs_point<T, MAG, ENT, DT, UPW, LAY>(d, P, w, S::ISS, S::IAX, &dt1, &u2, &b2,
+                                         LAY ? P.zlay + 4L * z : nullptr);
 
     if (a.mode == MODE_UPDATE) {
       const double dtb = a.d_dt ? a.d_dt[0] * a.beta : a.dtb;

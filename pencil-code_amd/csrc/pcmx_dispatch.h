@@ -11,10 +11,10 @@
 
 namespace pcmx {
 
-template <typename T, bool MAG, bool ENT, bool DT, int UPW>
+template <typename T, bool MAG, bool ENT, bool DT, int UPW, bool LAY = false>
 void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
   dim3 blk(64, 4, 1), grd((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
-  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT, UPW>), grd, blk, 0, s, a, P);
+  hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT, UPW, LAY>), grd, blk, 0, s, a, P);
 }
 
 template <typename T, bool MAG, bool ENT, class TL, int UPW>
@@ -30,10 +30,37 @@ bool launch_split(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s)
   }
 }
 
-template <typename T, bool MAG, bool ENT, int UPW>
+template <typename T, bool MAG, bool ENT, int UPW, bool LAY = false>
 void launch_direct_dt(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
-  if (c.dt) launch_direct<T, MAG, ENT, true, UPW>(a, P, s);
-  else launch_direct<T, MAG, ENT, false, UPW>(a, P, s);
+  if (c.dt) launch_direct<T, MAG, ENT, true, UPW, LAY>(a, P, s);
+  else launch_direct<T, MAG, ENT, false, UPW, LAY>(a, P, s);
+}
+
+// The layer physics (c.lay) runs on the direct kernel with the upwinding
+// masks of samples/conv-slab and its neighbours: none, lnrho, lnrho + ss,
+// uu + lnrho + ss with entropy; none or uu + lnrho without (lay_upw_ok).
+template <typename T, bool MAG, bool ENT>
+bool launch_direct_lay(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  switch (c.upw) {
+    case 0: launch_direct_dt<T, MAG, ENT, 0, true>(c, a, P, s); return true;
+    default: break;
+  }
+  if constexpr (ENT) {
+    switch (c.upw) {
+      case UPW_LNRHO: launch_direct_dt<T, MAG, ENT, UPW_LNRHO, true>(c, a, P, s); return true;
+      case UPW_LNRHO | UPW_SS: launch_direct_dt<T, MAG, ENT, UPW_LNRHO | UPW_SS, true>(c, a, P, s); return true;
+      case UPW_UU | UPW_LNRHO | UPW_SS:
+        launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_LNRHO | UPW_SS, true>(c, a, P, s);
+        return true;
+      default: return false;
+    }
+  } else {
+    if (c.upw == (UPW_UU | UPW_LNRHO)) {
+      launch_direct_dt<T, MAG, ENT, UPW_UU | UPW_LNRHO, true>(c, a, P, s);
+      return true;
+    }
+    return false;
+  }
 }
 
 // Upwinding: every mask in the direct kernel; in the split kernel uu and
@@ -42,6 +69,7 @@ void launch_direct_dt(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStre
 // z-marching kernel.
 template <typename T, bool MAG, bool ENT>
 bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
+  if (c.lay) return c.kernel == 1 && launch_direct_lay<T, MAG, ENT>(c, a, P, s);
   if (c.kernel == 3) {
     if (c.upw == 0) {
       if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16, 0>(a, P, c.zp, s);

@@ -315,6 +315,8 @@ int pcmx_ens_create(const pcmx_params* par, const pcmx_box* boxes, int nbox, int
   if (par->lupw_uu || par->lupw_lnrho) return fail(PCMX_ERR_ARG, "the ensemble does not provide upwinding");
   if (!par->lperi_z || par->gravz != 0.0)
     return fail(PCMX_ERR_ARG, "the ensemble runs periodic boxes without gravity");
+  if (layers_on(*par))
+    return fail(PCMX_ERR_ARG, "the ensemble does not provide rotation, K-profile conduction, heating or cooling");
   if (ge) {
     free_ens(ge);
     ge = nullptr;

@@ -70,10 +70,32 @@ int check_params(const pcmx_params* p) {
     if (p->nz < 8) return fail(PCMX_ERR_ARG, "a non-periodic z needs nz >= 8 (nz=%d)", p->nz);
     if (p->kernel == 2) return fail(PCMX_ERR_ARG, "kernel 2 runs periodic z only");
     for (int v = 0; v < p->mvar; ++v)
-      for (int c : {p->bcz_bot[v], p->bcz_top[v]})
-        if (c != PCMX_BC_S && c != PCMX_BC_A && c != PCMX_BC_A2)
-          return fail(PCMX_ERR_ARG, "bcz of variable %d: code %d is not PCMX_BC_S, _A or _A2", v, c);
+      for (int top = 0; top < 2; ++top) {
+        const int c = top ? p->bcz_top[v] : p->bcz_bot[v];
+        if (c == PCMX_BC_S || c == PCMX_BC_A || c == PCMX_BC_A2) continue;
+        if (c != PCMX_BC_C1 && c != PCMX_BC_CT)
+          return fail(PCMX_ERR_ARG, "bcz of variable %d: code %d is not a PCMX_BC_* code", v, c);
+        if (v != p->iss) return fail(PCMX_ERR_ARG, "bcz 'c1' and 'cT' apply to the entropy slot only");
+        if (c == PCMX_BC_C1 && !p->lheatc_kprof)
+          return fail(PCMX_ERR_ARG, "bcz 'c1' needs the K-profile heat conduction (lheatc_kprof)");
+        if (c == PCMX_BC_CT && !((top ? p->cs2top : p->cs2bot) > 0))
+          return fail(PCMX_ERR_ARG, "bcz 'cT' needs cs2%s > 0", top ? "top" : "bot");
+      }
   }
+  // layer physics (samples/conv-slab)
+  if (!std::isfinite(p->omega) || !std::isfinite(p->theta)) return fail(PCMX_ERR_ARG, "Omega or theta not finite");
+  if ((p->lheatc_kprof != 0 && p->lheatc_kprof != 1) || (p->cooling_profile != 0 && p->cooling_profile != 1))
+    return fail(PCMX_ERR_ARG, "lheatc_kprof is 0 or 1, cooling_profile 0 ('gaussian') or 1 ('step')");
+  const bool heatcool = p->luminosity != 0.0 || p->cool != 0.0;
+  if ((p->lheatc_kprof || heatcool) && !ent)
+    return fail(PCMX_ERR_ARG, "K-profile conduction, heating and cooling need the entropy variable");
+  if (p->lheatc_kprof && !(p->widthss != 0.0 && std::isfinite(p->hcond0)))
+    return fail(PCMX_ERR_ARG, "K-profile conduction needs widthss /= 0");
+  if (heatcool && !(p->cs2cool > 0)) return fail(PCMX_ERR_ARG, "heating or cooling needs cs2cool > 0");
+  if (p->luminosity != 0.0 && !(p->wheat > 0)) return fail(PCMX_ERR_ARG, "heating needs wheat > 0");
+  if (p->cool != 0.0 && !(p->wcool > 0)) return fail(PCMX_ERR_ARG, "cooling needs wcool > 0");
+  if (layers_on(*p) && !(p->lxyz[0] > 0 && p->lxyz[1] > 0 && p->lxyz[2] > 0))
+    return fail(PCMX_ERR_ARG, "the layer physics needs the box size lxyz");
   if (p->kernel < 0 || p->kernel > 3) return fail(PCMX_ERR_ARG, "unknown kernel %d", p->kernel);
   if ((p->lupw_uu != 0 && p->lupw_uu != 1) || (p->lupw_lnrho != 0 && p->lupw_lnrho != 1) ||
       (p->lupw_ss != 0 && p->lupw_ss != 1))
@@ -85,7 +107,80 @@ int check_params(const pcmx_params* p) {
   if (p->kernel == 3 && !split_upw_ok(ent, mask, TILE_32x8))
     return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu+lnrho (isothermal), lnrho+ss or uu+lnrho+ss (entropy); "
                 "use kernel 1 for other upwinding flags");
+  if (layers_on(*p)) {
+    if (p->kernel == 2 || p->kernel == 3)
+      return fail(PCMX_ERR_ARG, "Omega, K-profile conduction, heating and cooling run on kernel 1");
+    if (!lay_upw_ok(ent, mask))
+      return fail(PCMX_ERR_ARG, "with the layer physics the upwinding flags are none or lnrho, lnrho+ss, "
+                  "uu+lnrho+ss (entropy), uu+lnrho (isothermal)");
+  }
   return PCMX_OK;
+}
+
+// Per z plane {hcond, dlnhcond/dz, profz_heat, profz_cool} of the layer
+// physics: get_gravz_heatcond (entropy.f90:7741-7769) with step / der_step
+// (sub.f90:4975-5015, tini = 5*tiny), profz_heat (:1374-1386), profz_cool
+// (:1319-1368) times cool.  z(n) = z0 + n*dz (non-periodic z) or
+// z0 + (n+1/2)*dz.
+void layer_row(const pcmx_params& p, int n, double out[4]) {
+  const double tini = 5 * 2.2250738585072014e-308;
+  auto step = [&](double x, double x0, double w) { return 0.5 * (1 + std::tanh((x - x0) / (w + tini))); };
+  auto der_step = [&](double x, double x0, double w) {
+    const double arg = std::fabs((x - x0) / (w + tini));
+    if (std::fabs(arg) >= 8.) return 2. / w * std::exp(-2. * std::fabs(arg));
+    const double c = std::cosh(arg);
+    return 0.5 / (w * (c * c));
+  };
+  const double z = p.z0 + (p.lperi_z ? n + 0.5 : (double)n) * p.dz;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (p.lheatc_kprof) {
+    const double hc = 1. + (p.hcond1 - 1.) * step(z, p.z1, -p.widthss) + (p.hcond2 - 1.) * step(z, p.z2, p.widthss);
+    out[1] = ((p.hcond1 - 1.) * der_step(z, p.z1, -p.widthss) + (p.hcond2 - 1.) * der_step(z, p.z2, p.widthss)) / hc;
+    out[0] = p.hcond0 * hc;
+  }
+  if (p.luminosity != 0.0) {
+    const double pi = 3.14159265358979323846264338327950;
+    const double t = (z - p.z0) / p.wheat;
+    double h = p.luminosity * std::exp(-0.5 * (t * t)) / (std::sqrt(pi / 2.) * p.wheat);
+    if (p.ny == 1 && p.nx > 1) h = h / p.lxyz[0];
+    else if (p.nx == 1 && p.ny > 1) h = h / p.lxyz[1];
+    else if (p.nx > 1 && p.ny > 1) h = h / (p.lxyz[0] * p.lxyz[1]);
+    out[2] = h;
+  }
+  if (p.cool != 0.0) {
+    double c;
+    if (p.cooling_profile == 1) {
+      c = step(z, p.z2, p.wcool);
+    } else {
+      const double t = ((p.z0 + p.lxyz[2]) - z) / p.wcool;
+      c = std::exp(-0.5 * (t * t));
+    }
+    out[3] = p.cool * c;
+  }
+}
+
+// Constants of the 'c1' and 'cT' entropy rules of k_zbc (bc_ss_flux,
+// bc_ss_temp_z; initialize_energy :851-873 for FbotKbot / FtopKtop;
+// dz2_bound, grid.f90:3149-3151).
+void zbc_entropy_constants(const pcmx_params& p, ZBC& b) {
+  b.iss = p.iss;
+  b.cs20 = p.cs0 * p.cs0;
+  b.gamma_m1 = p.gamma - 1.0;
+  b.cv1 = p.gamma * (1 / p.cp);
+  b.lnrho0 = std::log(p.rho0);
+  b.cp = p.cp;
+  const double cv = (1 / p.gamma) * p.cp;
+  b.cpmcv = p.cp - cv;
+  const double kb = p.hcond0 * p.hcond1, kt = p.hcond0 * p.hcond2;
+  b.fk[0] = kb != 0.0 ? p.fbot / kb : 0.0;
+  b.fk[1] = kt != 0.0 ? p.ftop / kt : 0.0;
+  b.tmpT[0] = p.cs2bot > 0 ? 2 * cv * std::log(p.cs2bot / b.cs20) : 0.0;
+  b.tmpT[1] = p.cs2top > 0 ? 2 * cv * std::log(p.cs2top / b.cs20) : 0.0;
+  const int n2 = p.nz - 1;
+  for (int i = 1; i <= 3; ++i) {
+    b.dz2[0][i - 1] = 2. * ((p.z0 + i * p.dz) - (p.z0 + 0 * p.dz));
+    b.dz2[1][i - 1] = 2. * ((p.z0 + n2 * p.dz) - (p.z0 + (n2 - i) * p.dz));
+  }
 }
 
 // Derived constants, computed in the reference's operation order
@@ -124,6 +219,20 @@ Phys make_phys(const pcmx_params& p) {
   P.cdtv = p.cdtv;
   P.polytropic = (p.iss < 0 && p.gamma != 1.0) ? 1 : 0;
   P.gravz = p.gravz;
+  // coriolis_cartesian (hydro.f90:6339-6413); theta = 0 gives s2 = 0, which
+  // the general form adds as +0
+  if (p.theta == 0.0) {
+    P.cor_c2 = 2 * p.omega;
+    P.cor_s2 = 0.0;
+  } else {
+    const double dtor = 3.14159265358979323846264338327950 / 180.;
+    P.cor_c2 = 2 * p.omega * std::cos(p.theta * dtor);
+    P.cor_s2 = -2 * p.omega * std::sin(p.theta * dtor);
+  }
+  P.cs2cool = p.cs2cool;
+  P.kprof = p.lheatc_kprof;
+  P.heatcool = (p.luminosity != 0.0 || p.cool != 0.0) ? 1 : 0;
+  P.zlay = nullptr;  // the engine's device table, set by the caller
   return P;
 }
 
@@ -166,6 +275,8 @@ struct Engine {
   // non-periodic z: ghost planes of the state ([v][6][ny][nx]) and the rules
   double* zgh = nullptr;
   ZBC zbc{};
+  // layer physics: per z plane {hcond, dlnhcond, profz_heat, profz_cool}
+  double* zlay = nullptr;
   // two fixed-dt steps captured as a HIP graph per [ldiag][cur], valid for
   // the dt and the parameter generation they were captured with
   hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -190,6 +301,7 @@ LaunchCfg cfg_of(const Engine& e, bool dt) {
   c.dt = dt;
   c.zp = e.zplan;
   c.upw = (e.par.lupw_uu ? UPW_UU : 0) | (e.par.lupw_lnrho ? UPW_LNRHO : 0) | (e.par.lupw_ss ? UPW_SS : 0);
+  c.lay = layers_on(e.par);
   return c;
 }
 
@@ -385,6 +497,7 @@ void free_engine(Engine* e) {
     if (p) (void)hipFree(p);
   if (e->ftab) (void)hipFree(e->ftab);
   if (e->zgh) (void)hipFree(e->zgh);
+  if (e->zlay) (void)hipFree(e->zlay);
   for (auto& row : e->gexec)
     for (auto& x : row)
       if (x) (void)hipGraphExecDestroy(x);
@@ -481,7 +594,7 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   // upwinding of uu alone, of lnrho alone, or with entropy: the direct kernel
   const int upw_mask = (par->lupw_uu ? UPW_UU : 0) | (par->lupw_lnrho ? UPW_LNRHO : 0) | (par->lupw_ss ? UPW_SS : 0);
   const bool upw_split = tile >= 0 && split_upw_ok(par->iss >= 0, upw_mask, tile);
-  if (kern == 0) kern = (tile >= 0 && upw_split) ? 3 : 1;
+  if (kern == 0) kern = (tile >= 0 && upw_split && !layers_on(*par)) ? 3 : 1;
   if (kern == 2 && !zmarch_supported(*par)) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
@@ -521,6 +634,21 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   for (int v = 0; v < 8; ++v) {
     e->zbc.bot[v] = par->bcz_bot[v];
     e->zbc.top[v] = par->bcz_top[v];
+  }
+  zbc_entropy_constants(*par, e->zbc);
+  if (layers_on(*par)) {
+    std::vector<double> tab(4 * (size_t)par->nz);
+    for (int n = 0; n < par->nz; ++n) layer_row(*par, n, &tab[4 * (size_t)n]);
+    if ((rc = alloc(&e->zlay, sizeof(double) * tab.size()))) {
+      free_engine(e);
+      return rc;
+    }
+    hipError_t cerr = hipMemcpy(e->zlay, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+    if (cerr != hipSuccess) {
+      free_engine(e);
+      return fail(PCMX_ERR_HIP, "hipMemcpy(layer table) failed: %s", hipGetErrorString(cerr));
+    }
+    e->phys.zlay = e->zlay;
   }
   hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (err != hipSuccess) {
@@ -696,6 +824,7 @@ int pcmx_update_by_index(int index, double value) {
   }
   if ((rc = check_params(&e.par))) return rc;
   e.phys = make_phys(e.par);
+  e.phys.zlay = e.zlay;
   ++e.gen;
   return PCMX_OK;
 }

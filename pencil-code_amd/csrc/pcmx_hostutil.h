@@ -23,6 +23,12 @@ int check_params(const pcmx_params* p);
 // Derived constants of one box, in the reference's operation order.
 Phys make_phys(const pcmx_params& p);
 
+// The layer physics of samples/conv-slab is on: Coriolis force, K-profile
+// conduction, heating or cooling.
+inline bool layers_on(const pcmx_params& p) {
+  return p.omega != 0.0 || p.lheatc_kprof != 0 || p.luminosity != 0.0 || p.cool != 0.0;
+}
+
 }  // namespace pcmx
 
 #define HIPCHECK(call)                                                                                  \

@@ -25,6 +25,7 @@ struct LaunchCfg {
   bool dt;      // kernels 1 and 2: evaluate the Courant terms
   ZPlan zp;     // kernels 2 and 3: the grid of tile columns
   int upw = 0;  // upwinding mask (UPW_UU, UPW_LNRHO, UPW_SS bits)
+  bool lay = false;  // layer physics of samples/conv-slab (direct kernel only)
 };
 
 struct EnsBox;
@@ -51,6 +52,12 @@ bool launch_ens_f32(const LaunchCfg& c, int stage, const EnsBox* boxes, const En
 inline bool split_upw_ok(bool ent, int mask, int tile) {
   return mask == 0 || (!ent && mask == (UPW_UU | UPW_LNRHO)) ||
          (ent && tile == TILE_32x8 && (mask == (UPW_LNRHO | UPW_SS) || mask == (UPW_UU | UPW_LNRHO | UPW_SS)));
+}
+
+// Upwinding masks the direct kernel provides with the layer physics.
+inline bool lay_upw_ok(bool ent, int mask) {
+  if (ent) return mask == 0 || mask == UPW_LNRHO || mask == (UPW_LNRHO | UPW_SS) || mask == (UPW_UU | UPW_LNRHO | UPW_SS);
+  return mask == 0 || mask == (UPW_UU | UPW_LNRHO);
 }
 
 inline int split_tile(int nx, int ny) {

@@ -32,6 +32,15 @@ struct Phys {
   double cdt, cdtv;
   double gravz;     // constant vertical gravity (gravity_simple, gravz_profile='const'), or 0
   int polytropic;   // noentropy with gamma /= 1 (ilnrho_ss, leos_isentropic)
+  // Layer physics of samples/conv-slab (kernels instantiated with LAY):
+  // Coriolis coefficients c2 = 2 Omega cos(theta), s2 = -2 Omega sin(theta)
+  // (coriolis_cartesian, hydro.f90:6339-6413); per z plane n the row
+  // zlay[4n..4n+3] = {hcond, dlnhcond/dz, profz_heat, profz_cool}
+  // (get_gravz_heatcond, entropy.f90:7741-7769; :1319-1386); cs2cool; kprof
+  // for iheatcond='K-profile', heatcool when luminosity or cool is set.
+  double cor_c2, cor_s2, cs2cool;
+  const double* zlay;
+  int kprof, heatcool;
 };
 
 // der2_coef*, deriv.f90:36-40 ('standard').
@@ -152,14 +161,16 @@ __device__ __forceinline__ T advec_va2_of(const T (&bb)[3], T rho1, const Phys& 
 }
 
 // dt1_max of one cell from the advective pencils (set_dt1_max).
-template <typename T, bool MAG, bool ENT>
-__device__ __forceinline__ double courant_dt1(T advec_uu, T cs2, T advec_va2, const Phys& P) {
+template <typename T, bool MAG, bool ENT, bool LAY = false>
+__device__ __forceinline__ double courant_dt1(T advec_uu, T cs2, T advec_va2, const Phys& P, T dchi = 0) {
   const T advec_cs2 = cs2 * (T)P.dxyz_2;
   const T advec2 = (MAG ? advec_va2 : (T)0) + advec_cs2;
   const T maxadvec = advec_uu + psqrt<T>(advec2);
   T maxdiffus = (T)P.nu * (T)P.dxyz_2;                                 // viscosity.f90:2435
   if constexpr (MAG) maxdiffus = fmax(maxdiffus, (T)P.eta * (T)P.dxyz_2);
-  if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
+  // diffus_chi: K-profile chix*dxyz_2 (entropy.f90:6094) + chi-const (:4907)
+  if constexpr (ENT && LAY) maxdiffus = fmax(maxdiffus, dchi + ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
+  else if constexpr (ENT) maxdiffus = fmax(maxdiffus, ((T)P.gamma * (T)P.chi) * (T)P.dxyz_2);
   const T dt1_advec = maxadvec / (T)P.cdt;
   const T dt1_diffus = maxdiffus / (T)P.cdtv;
   return (double)psqrt<T>(dt1_advec * dt1_advec + dt1_diffus * dt1_diffus);
@@ -185,6 +196,7 @@ struct FieldOut {
 template <typename T>
 struct FlowKeep {
   T rho1, TT1, cs2, advec_uu;
+  T dchi;  // K-profile diffusion chix*dxyz_2 (LAY only)
 };
 
 // Magnetic pencils (magnetic.f90:4279-5225; sub.f90:2945-3113) and daa_dt
@@ -235,9 +247,9 @@ __device__ __forceinline__ void field_point(const Derivs<T>& d, T rho1, const Ph
 // argument so that both halves use the same value.  Without entropy the EOS
 // is isothermal (gamma = 1) or, with POLY, polytropic (noentropy.f90:94-103);
 // it is a compile-time choice so that no branch splits the algebra.
-template <typename T, bool ENT, bool DT, bool POLY, int UPW = 0>
+template <typename T, bool ENT, bool DT, bool POLY, int UPW = 0, bool LAY = false>
 __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const Phys& P, double* wf,
-                                               FlowKeep<T>& k, double* u2_out) {
+                                               FlowKeep<T>& k, double* u2_out, const double* zl = nullptr) {
   // ---- hydro pencils (hydro.f90:3925-4000; sub.f90 helpers) ----
   const T u0 = d.u[0], u1 = d.u[1], u2v = d.u[2];
   const T u2 = u0 * u0 + u1 * u1 + u2v * u2v;                           // dot2_mn
@@ -308,7 +320,12 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
   for (int j = 0; j < 3; ++j) {
     double acc = wf[j];
     acc = acc - (double)ugu[j];        // hydro.f90:4384
-    acc = acc + (double)fvisc[j];      // viscosity.f90:2823
+    if constexpr (LAY) {               // Coriolis force (hydro.f90:4494, :6339-6413)
+      if (j == 0) acc = acc + (double)((T)P.cor_c2 * u1);
+      else if (j == 1) acc = (acc - (double)((T)P.cor_c2 * u0)) + (double)((T)P.cor_s2 * u2v);
+      else acc = acc - (double)((T)P.cor_s2 * u1);
+    }
+    acc = acc + (double)fvisc[j];      // viscosity.f90:2823 (calc_viscous_force, hydro.f90:4518)
     acc = acc + (double)fpres[j];      // noentropy.f90:381 / entropy.f90:3714
     wf[j] = acc;
   }
@@ -321,10 +338,31 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
     double acc = wf[4];
     acc = acc - (double)ugss;                               // entropy.f90:3758
     acc = acc + (double)(TT1 * visc_heat);                  // viscosity.f90:2764
+    if constexpr (LAY) {
+      // K-profile conduction (entropy.f90:3800, :6046-6067): glnThcond =
+      // glnTT + (0, 0, dlnhcond); thdiff = rho1*hcond*(del2lnTT + glnTT.glnThcond)
+      if (P.kprof) {
+        const T hc = (T)zl[0];
+        const T gk = glnTT[0] * glnTT[0] + glnTT[1] * glnTT[1] + glnTT[2] * (glnTT[2] + (T)zl[1]);
+        acc = acc + (double)(rho1 * hc * (del2lnTT + gk));
+      }
+    }
     const T a0 = d.glnrho[0] + glnTT[0], a1 = d.glnrho[1] + glnTT[1], a2 = d.glnrho[2] + glnTT[2];
     const T g2 = a0 * glnTT[0] + a1 * glnTT[1] + a2 * glnTT[2];
     acc = acc + (double)((T)P.chi * (T)P.cp * (del2lnTT + g2));  // entropy.f90:4852-4947
+    if constexpr (LAY) {
+      // heating and cooling (calc_heat_cool :3839, :6554; get_heat_cool_gravz
+      // :6761-6828): heat = profz_heat - profz_cool*(cs2-cs2cool)/cs2cool
+      if (P.heatcool) {
+        const T heat = (T)zl[2] - (T)zl[3] * (cs2 - (T)P.cs2cool) / (T)P.cs2cool;
+        acc = acc + (double)(TT1 * rho1 * heat);
+      }
+    }
     wf[4] = acc;
+  }
+  k.dchi = 0;
+  if constexpr (LAY && ENT && DT) {
+    if (P.kprof) k.dchi = rho1 * (T)zl[0] * (T)P.cv1 * (T)P.dxyz_2;  // chix*dxyz_2 (entropy.f90:6094)
   }
   k.rho1 = rho1;
   k.TT1 = TT1;
@@ -337,7 +375,7 @@ __device__ __forceinline__ void flow_point_pre(const Derivs<T>& d, T rho1, const
 // The magnetic contributions to the flow RHS (Lorentz force last in duu_dt,
 // magnetic.f90:5531; ohmic heating last in the entropy equation,
 // magnetic.f90:6143) and set_dt1_max (equ.f90:1858-1912).
-template <typename T, bool MAG, bool ENT, bool DT>
+template <typename T, bool MAG, bool ENT, bool DT, bool LAY = false>
 __device__ __forceinline__ void flow_point_post(const FlowKeep<T>& k, const FieldOut<T>& fo, const Phys& P,
                                                 double* wf, double* dt1) {
   if constexpr (MAG) {
@@ -348,14 +386,15 @@ __device__ __forceinline__ void flow_point_post(const FlowKeep<T>& k, const Fiel
   // addgravity after the magnetic terms (equ.f90:1316; gravity_simple.f90:1083-1162);
   // adding 0 without gravity leaves the value unchanged
   wf[2] = wf[2] + P.gravz;
-  if constexpr (DT) *dt1 = courant_dt1<T, MAG, ENT>(k.advec_uu, k.cs2, fo.advec_va2, P);
+  if constexpr (DT) *dt1 = courant_dt1<T, MAG, ENT, LAY>(k.advec_uu, k.cs2, fo.advec_va2, P, k.dchi);
 }
 
 // Whole RHS of one cell: adds it to w[] (already alpha-scaled; slots 0..2 uu,
 // 3 lnrho, [iss], [iax..iax+2]) and returns dt1_max_loc (when DT), u2 and b2.
-template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0>
+template <typename T, bool MAG, bool ENT, bool DT, int UPW = 0, bool LAY = false>
 __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, double* w, int iss,
-                                          int iax, double* dt1, double* u2_out, double* b2_out) {
+                                          int iax, double* dt1, double* u2_out, double* b2_out,
+                                          const double* zl = nullptr) {
   const T rho1 = pexp<T>(-d.lnrho);
   FieldOut<T> fo{};
   *b2_out = 0.0;
@@ -365,9 +404,9 @@ __device__ __forceinline__ void rhs_point(const Derivs<T>& d, const Phys& P, dou
   for (int v = 0; v < 4; ++v) wf[v] = w[v];
   if constexpr (ENT) wf[4] = w[iss];
   FlowKeep<T> k;
-  if (!ENT && P.polytropic) flow_point_pre<T, ENT, DT, true, UPW>(d, rho1, P, wf, k, u2_out);
-  else flow_point_pre<T, ENT, DT, false, UPW>(d, rho1, P, wf, k, u2_out);
-  flow_point_post<T, MAG, ENT, DT>(k, fo, P, wf, dt1);
+  if (!ENT && P.polytropic) flow_point_pre<T, ENT, DT, true, UPW, LAY>(d, rho1, P, wf, k, u2_out, zl);
+  else flow_point_pre<T, ENT, DT, false, UPW, LAY>(d, rho1, P, wf, k, u2_out, zl);
+  flow_point_post<T, MAG, ENT, DT, LAY>(k, fo, P, wf, dt1);
 #pragma unroll
   for (int v = 0; v < 4; ++v) w[v] = wf[v];
   if constexpr (ENT) w[iss] = wf[4];

@@ -178,6 +178,16 @@ contains
         enddo
       endif
 !
+!  The layer physics of samples/conv-slab (rotation, K-profile conduction,
+!  heating and cooling, bcz 'c1' / 'cT') is not mapped from this module's
+!  reference modules: those runs keep the CPU path.
+!
+      par%omega = 0.; par%theta = 0.; par%lheatc_kprof = 0; par%cooling_profile = 0
+      par%hcond0 = 0.; par%hcond1 = 0.; par%hcond2 = 0.; par%widthss = 0.; par%z1 = 0.; par%z2 = 0.
+      par%luminosity = 0.; par%wheat = 0.; par%cool = 0.; par%wcool = 0.; par%cs2cool = 0.
+      par%fbot = 0.; par%ftop = 0.; par%cs2bot = 0.; par%cs2top = 0.
+      par%z0 = xyz0(3); par%lxyz = Lxyz
+!
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)
       if (ltest_bcs) call test_gpu_bcs

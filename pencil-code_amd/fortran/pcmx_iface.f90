@@ -29,10 +29,18 @@ module pcmx_iface
     integer(c_int) :: lperi_z
     integer(c_int) :: bcz_bot(8), bcz_top(8)
     integer(c_int) :: lupw_ss
+    ! layer physics of samples/conv-slab (direct kernel)
+    real(c_double) :: omega, theta
+    integer(c_int) :: lheatc_kprof, cooling_profile
+    real(c_double) :: hcond0, hcond1, hcond2, widthss, z1, z2
+    real(c_double) :: luminosity, wheat, cool, wcool, cs2cool
+    real(c_double) :: fbot, ftop, cs2bot, cs2top
+    real(c_double) :: z0, lxyz(3)
   end type pcmx_params
 
   ! z boundary codes of pcmx_params (PCMX_BC_* of include/pcmx.h)
   integer(c_int), parameter :: PCMX_BC_S = 1, PCMX_BC_A = 2, PCMX_BC_A2 = 3
+  integer(c_int), parameter :: PCMX_BC_C1 = 4, PCMX_BC_CT = 5
 
   ! pcmx_forcing of include/pcmx.h: helical forcing after each step
   type, bind(C) :: pcmx_forcing

@@ -59,6 +59,11 @@ program pcmx_run
   par%ldt = merge(1, 0, ldt); par%precision = precision; par%kernel = kernel
   par%lupw_uu = 0; par%lupw_lnrho = 0
   par%gravz = 0.; par%lperi_z = 1; par%bcz_bot = 0; par%bcz_top = 0; par%lupw_ss = 0
+  par%omega = 0.; par%theta = 0.; par%lheatc_kprof = 0; par%cooling_profile = 0
+  par%hcond0 = 0.; par%hcond1 = 0.; par%hcond2 = 0.; par%widthss = 0.; par%z1 = 0.; par%z2 = 0.
+  par%luminosity = 0.; par%wheat = 0.; par%cool = 0.; par%wcool = 0.; par%cs2cool = 0.
+  par%fbot = 0.; par%ftop = 0.; par%cs2bot = 0.; par%cs2top = 0.
+  par%z0 = -Lz/2; par%lxyz = (/ Lx, Ly, Lz /)
 
   t = 0d0
   call check(pcmx_register(), 'register')

@@ -52,10 +52,46 @@ class Params(ctypes.Structure):
         ("gravz", ctypes.c_double), ("lperi_z", ctypes.c_int),
         ("bcz_bot", ctypes.c_int * 8), ("bcz_top", ctypes.c_int * 8),
         ("lupw_ss", ctypes.c_int),
+        ("omega", ctypes.c_double), ("theta", ctypes.c_double),
+        ("lheatc_kprof", ctypes.c_int), ("cooling_profile", ctypes.c_int),
+        ("hcond0", ctypes.c_double), ("hcond1", ctypes.c_double), ("hcond2", ctypes.c_double),
+        ("widthss", ctypes.c_double), ("z1", ctypes.c_double), ("z2", ctypes.c_double),
+        ("luminosity", ctypes.c_double), ("wheat", ctypes.c_double),
+        ("cool", ctypes.c_double), ("wcool", ctypes.c_double), ("cs2cool", ctypes.c_double),
+        ("fbot", ctypes.c_double), ("ftop", ctypes.c_double),
+        ("cs2bot", ctypes.c_double), ("cs2top", ctypes.c_double),
+        ("z0", ctypes.c_double), ("lxyz", ctypes.c_double * 3),
     ]
 
 
-BC_CODES = {"s": 1, "a": 2, "a2": 3}  # PCMX_BC_S, _A, _A2
+BC_CODES = {"s": 1, "a": 2, "a2": 3, "c1": 4, "cT": 5}  # PCMX_BC_S, _A, _A2, _C1, _CT
+
+# layer fields of Params (samples/conv-slab physics), set through make_params(layers=...)
+LAYER_FIELDS = ("lheatc_kprof", "cooling_profile", "hcond0", "hcond1", "hcond2", "widthss", "z1", "z2",
+                "luminosity", "wheat", "cool", "wcool", "cs2cool", "fbot", "ftop", "cs2bot", "cs2top")
+
+
+def layer_defaults(gamma, cp, gravz, bcz_ss, hcond0=0.0, mpoly0=1.5, mpoly1=1.5, mpoly2=1.5, widthss=0.05,
+                   z1=0.0, z2=1.0, luminosity=0.0, wheat=0.1, cool=0.0, wcool=0.1, cs2cool=0.0,
+                   cooling_profile="gaussian", cs2top=None, cs2bot=None, fbot=None, ftop=None, kprof=True):
+    """Layer fields derived as initialize_energy does (entropy.f90:805-890):
+    hcond1 = (mpoly1+1)/(mpoly0+1), hcond2 = (mpoly2+1)/(mpoly0+1) (the
+    K-profile of a piecewise polytrope); Fbot (Ftop) =
+    -gamma/(cp*(gamma-1))*hcond0*gravz/(mpoly0+1) when the bottom (top) rule
+    of ss is 'c1', else 0, unless given; with cooling, cs2top =
+    (cool*cs2cool + cool2*cs2cool2)/(cool+cool2) with cool2 = cs2cool2 = 0,
+    unless given.  bcz_ss: the run.in entry of ss, e.g. 'c1:cT'."""
+    lo, _, hi = bcz_ss.partition(":")
+    hi = hi or lo
+    flux = -gamma / (cp * (gamma - 1.0)) * hcond0 * gravz / (mpoly0 + 1)
+    if cs2top is None:
+        cs2top = (cool * cs2cool + 0.0 * 0.0) / (cool + 0.0) if cool != 0.0 else 0.0
+    return dict(lheatc_kprof=int(kprof), cooling_profile={"gaussian": 0, "step": 1}[cooling_profile],
+                hcond0=hcond0, hcond1=(mpoly1 + 1.) / (mpoly0 + 1.), hcond2=(mpoly2 + 1.) / (mpoly0 + 1.),
+                widthss=widthss, z1=z1, z2=z2, luminosity=luminosity, wheat=wheat, cool=cool, wcool=wcool,
+                cs2cool=cs2cool, fbot=(flux if lo == "c1" else 0.0) if fbot is None else fbot,
+                ftop=(flux if hi == "c1" else 0.0) if ftop is None else ftop,
+                cs2bot=0.0 if cs2bot is None else cs2bot, cs2top=cs2top)
 
 
 def bc_codes(bcz, nv):
@@ -96,15 +132,17 @@ ENS_NO_GRAPH = 1
 def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, cs0=1.0, gamma=1.0,
                 cp=1.0, rho0=1.0, mu0=1.0, cdt=0.9, cdtv=0.25, ldt=False, precision=PREC_FP64,
                 kernel=None, L=2 * math.pi, lupw_uu=False, lupw_lnrho=False, gravz=0.0, bcz=None,
-                lupw_ss=False):
+                lupw_ss=False, omega=0.0, theta=0.0, layers=None, z0=None):
     """Parameters of one periodic box with the reference defaults
     (cdata.f90:235-239, eos_idealgas.f90:40-46).  Slots follow the
     registration order uu, lnrho, [ss], [aa] (register.f90:137-146).
     kernel None takes PCMX_KERNEL from the environment (0 auto, 1 direct,
     2 z-marching, 3 z-marching with the flow and field halves on separate
     wavefronts), default auto.  bcz: None for a periodic z, else one run.in
-    entry per variable ('s', 'a', 'a2', 'bottom:top'); the z spacing is then
-    Lz/(nz-1), boundary points on the first and last plane."""
+    entry per variable ('s', 'a', 'a2', 'c1', 'cT', 'bottom:top'); the z
+    spacing is then Lz/(nz-1), boundary points on the first and last plane.
+    omega, theta: rotation; layers: the conv-slab fields (layer_defaults);
+    z0: z of the first grid point (default -Lz/2)."""
     if kernel is None:
         kernel = int(os.environ.get("PCMX_KERNEL", KERNEL_AUTO))
     nx, ny, nz = (n, n, n) if np.isscalar(n) else n
@@ -114,9 +152,17 @@ def make_params(n, lmagnetic=False, lentropy=False, nu=5e-3, eta=0.0, chi=0.0, c
     mvar = 4 + (1 if lentropy else 0) + (3 if lmagnetic else 0)
     bot, top = bc_codes(bcz, mvar) if bcz is not None else ((ctypes.c_int * 8)(), (ctypes.c_int * 8)())
     dz = Lz / nz if bcz is None else Lz / (nz - 1)
-    return Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, dz, nu, eta, chi, cs0, gamma,
-                  cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho),
-                  gravz, int(bcz is None), bot, top, int(lupw_ss))
+    p = Params(nx, ny, nz, mvar, 0, 3, iss, iax, Lx / nx, Ly / ny, dz, nu, eta, chi, cs0, gamma,
+               cp, rho0, mu0, cdt, cdtv, int(ldt), precision, kernel, int(lupw_uu), int(lupw_lnrho),
+               gravz, int(bcz is None), bot, top, int(lupw_ss))
+    p.omega, p.theta = omega, theta
+    p.z0 = -Lz / 2 if z0 is None else z0
+    p.lxyz[:] = (Lx, Ly, Lz)
+    for k, v in (layers or {}).items():
+        if k not in LAYER_FIELDS:
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
 
 
 _lib = None

@@ -20,7 +20,7 @@ module Cdata
   logical :: loutput_varn_at_exact_tsnap=.false.
   logical, dimension(3) :: lperi=.true., lequidist=.true.
   integer :: iux=1, ilnrho=4, iss=0, iax=5, itorder=3, nt=10
-  real, dimension(3) :: xyz0=-3.14159
+  real, dimension(3) :: xyz0=-3.14159, Lxyz=6.28318
   integer :: n1=4
   character(len=bclen), dimension(mcom,2) :: bcz12=''
 end module Cdata
