@@ -68,8 +68,9 @@ typedef struct pcmx_params {
   int bcz_bot[8], bcz_top[8];/* run_pars bcz per variable slot when lperi_z == 0: PCMX_BC_* */
   int lupw_ss;               /* entropy_run_pars lupw_ss: upwinded u.grad ss (entropy.f90:3576);
                                 entropy runs, direct kernel */
-  /* Layer physics of samples/conv-slab, on the direct kernel (kernel 1; kernel 0
-   * selects it when any of these is on).  All zero: off. */
+  /* Layer physics of samples/conv-slab: on the split kernel (kernel 3) with entropy,
+   * lupw_lnrho = lupw_ss = 1 and 32x8 tiles, else on the direct kernel (kernel 1);
+   * kernel 0 chooses.  All zero: off. */
   double omega, theta;       /* hydro_run_pars Omega, theta (degrees): Coriolis force
                                 (coriolis_cartesian, hydro.f90:6339-6413) */
   int lheatc_kprof;          /* entropy_run_pars iheatcond='K-profile' (lmultilayer, lgravz):

@@ -17,15 +17,15 @@ void launch_direct(const KArgs& a, const Phys& P, hipStream_t s) {
   hipLaunchKernelGGL((k_rhs_direct<T, MAG, ENT, DT, UPW, LAY>), grd, blk, 0, s, a, P);
 }
 
-template <typename T, bool MAG, bool ENT, class TL, int UPW>
+template <typename T, bool MAG, bool ENT, class TL, int UPW, bool LAY = false>
 bool launch_split(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
   switch (a.stage) {
-    case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1, TL, UPW>(a, P, zp, s); return true;
-    case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2, TL, UPW>(a, P, zp, s); return true;
-    case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3, TL, UPW>(a, P, zp, s); return true;
-    case ST_W: launch_zsplit<T, MAG, ENT, ST_W, TL, UPW>(a, P, zp, s); return true;
-    case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D, TL, UPW>(a, P, zp, s); return true;
-    case ST_DT: launch_zsplit<T, MAG, ENT, ST_DT, TL, UPW>(a, P, zp, s); return true;
+    case ST_S1: launch_zsplit<T, MAG, ENT, ST_S1, TL, UPW, LAY>(a, P, zp, s); return true;
+    case ST_S2: launch_zsplit<T, MAG, ENT, ST_S2, TL, UPW, LAY>(a, P, zp, s); return true;
+    case ST_S3: launch_zsplit<T, MAG, ENT, ST_S3, TL, UPW, LAY>(a, P, zp, s); return true;
+    case ST_W: launch_zsplit<T, MAG, ENT, ST_W, TL, UPW, LAY>(a, P, zp, s); return true;
+    case ST_S2D: launch_zsplit<T, MAG, ENT, ST_S2D, TL, UPW, LAY>(a, P, zp, s); return true;
+    case ST_DT: launch_zsplit<T, MAG, ENT, ST_DT, TL, UPW, LAY>(a, P, zp, s); return true;
     default: return false;
   }
 }
@@ -69,7 +69,16 @@ bool launch_direct_lay(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStr
 // z-marching kernel.
 template <typename T, bool MAG, bool ENT>
 bool launch_phys(const LaunchCfg& c, const KArgs& a, const Phys& P, hipStream_t s) {
-  if (c.lay) return c.kernel == 1 && launch_direct_lay<T, MAG, ENT>(c, a, P, s);
+  if (c.lay) {
+    if (c.kernel == 3) {  // entropy, lnrho + ss upwinding, 32x8 tiles (split_lay_ok)
+      if constexpr (ENT) {
+        constexpr int M6 = UPW_LNRHO | UPW_SS;
+        if (c.tile == TILE_32x8 && c.upw == M6) return launch_split<T, MAG, ENT, ZT32x8, M6, true>(a, P, c.zp, s);
+      }
+      return false;
+    }
+    return c.kernel == 1 && launch_direct_lay<T, MAG, ENT>(c, a, P, s);
+  }
   if (c.kernel == 3) {
     if (c.upw == 0) {
       if (c.tile == TILE_16x16) return launch_split<T, MAG, ENT, ZT16x16, 0>(a, P, c.zp, s);

@@ -108,8 +108,9 @@ int check_params(const pcmx_params* p) {
     return fail(PCMX_ERR_ARG, "kernel 3 upwinds uu+lnrho (isothermal), lnrho+ss or uu+lnrho+ss (entropy); "
                 "use kernel 1 for other upwinding flags");
   if (layers_on(*p)) {
-    if (p->kernel == 2 || p->kernel == 3)
-      return fail(PCMX_ERR_ARG, "Omega, K-profile conduction, heating and cooling run on kernel 1");
+    if (p->kernel == 2 || (p->kernel == 3 && !split_lay_ok(ent, mask, TILE_32x8)))
+      return fail(PCMX_ERR_ARG, "Omega, K-profile conduction, heating and cooling run on kernel 1, or on "
+                  "kernel 3 with entropy, lnrho+ss upwinding and 32x8 tiles");
     if (!lay_upw_ok(ent, mask))
       return fail(PCMX_ERR_ARG, "with the layer physics the upwinding flags are none or lnrho, lnrho+ss, "
                   "uu+lnrho+ss (entropy), uu+lnrho (isothermal)");
@@ -593,8 +594,9 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   const int tile = (par->nz >= 4) ? split_tile(par->nx, par->ny) : -1;
   // upwinding of uu alone, of lnrho alone, or with entropy: the direct kernel
   const int upw_mask = (par->lupw_uu ? UPW_UU : 0) | (par->lupw_lnrho ? UPW_LNRHO : 0) | (par->lupw_ss ? UPW_SS : 0);
-  const bool upw_split = tile >= 0 && split_upw_ok(par->iss >= 0, upw_mask, tile);
-  if (kern == 0) kern = (tile >= 0 && upw_split && !layers_on(*par)) ? 3 : 1;
+  const bool upw_split = tile >= 0 && (layers_on(*par) ? split_lay_ok(par->iss >= 0, upw_mask, tile)
+                                                        : split_upw_ok(par->iss >= 0, upw_mask, tile));
+  if (kern == 0) kern = (tile >= 0 && upw_split) ? 3 : 1;
   if (kern == 2 && !zmarch_supported(*par)) {
     delete e;
     return fail(PCMX_ERR_ARG, "z-marching kernel needs nx %% %d == 0 and ny %% %d == 0", ZM_TX, ZM_TY);
@@ -605,7 +607,8 @@ int pcmx_initialize(const double* f, int mfarray, const pcmx_params* par, double
   }
   if (kern == 3 && !upw_split) {
     delete e;
-    return fail(PCMX_ERR_ARG, "kernel 3 upwinds with entropy on 32x8 tiles only (nx %% 32 == 0, ny %% 8 == 0); "
+    return fail(PCMX_ERR_ARG, "kernel 3 upwinds with entropy, and runs the layer physics, on 32x8 tiles only "
+                "(nx %% 32 == 0, ny %% 8 == 0); "
                 "use kernel 1");
   }
   e->kernel = kern;
@@ -904,7 +907,8 @@ int pcmx_tune(double* best_ms) {
   const int upw_mask = cfg_of(e, false).upw;
   for (int tile : {TILE_32x8, TILE_16x16}) {
     if (e.par.nx % tile_tx(tile) || e.par.ny % tile_ty(tile)) continue;
-    if (!split_upw_ok(e.ent, upw_mask, tile)) continue;  // no variant for this tile
+    const bool ok = layers_on(e.par) ? split_lay_ok(e.ent, upw_mask, tile) : split_upw_ok(e.ent, upw_mask, tile);
+    if (!ok) continue;  // no variant for this tile
     for (int lz : {8, 16, 32, 64, 128, 256, 512, 1024})
       if (lz <= e.par.nz && e.par.nz % lz == 0) cands.push_back({tile, lz});
     if (e.par.nz > 1024 || e.par.nz % 8) cands.push_back({tile, e.par.nz});

@@ -54,6 +54,12 @@ inline bool split_upw_ok(bool ent, int mask, int tile) {
          (ent && tile == TILE_32x8 && (mask == (UPW_LNRHO | UPW_SS) || mask == (UPW_UU | UPW_LNRHO | UPW_SS)));
 }
 
+// The split kernel carries the layer physics for the samples/conv-slab
+// choice: entropy, lnrho + ss upwinding, 32x8 tiles.
+inline bool split_lay_ok(bool ent, int mask, int tile) {
+  return ent && tile == TILE_32x8 && mask == (UPW_LNRHO | UPW_SS);
+}
+
 // Upwinding masks the direct kernel provides with the layer physics.
 inline bool lay_upw_ok(bool ent, int mask) {
   if (ent) return mask == 0 || mask == UPW_LNRHO || mask == (UPW_LNRHO | UPW_SS) || mask == (UPW_UU | UPW_LNRHO | UPW_SS);

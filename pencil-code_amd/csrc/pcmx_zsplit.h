@@ -211,7 +211,8 @@ constexpr int zs_xch_elems() {
 // Marches one tile column (x0, y0, z0 .. z0+lz-1) of one box through the
 // RK stage ST.  ring and xch are the block's LDS buffers; the column's
 // reduction terms are added to red (all threads of the block take part).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS, int RING, int UPW = 0>
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, bool PAIRS, int RING, int UPW = 0,
+          bool LAY = false>
 __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int x0, int y0, int z0, int lz,
                                               T* ring, T* xch, double (&red)[NRED]) {
   using S = Slots<ENT, MAG>;
@@ -678,7 +679,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 #pragma unroll
         for (int i = 0; i < NF; ++i) wf[i] = G::READW ? a.alpha * wcur[i] : 0.0;
         const T rho1 = pexp<T>(-d.lnrho);
-        flow_point_pre<T, ENT, DT, POLY, UPW>(d, rho1, P, wf, keep, &u2c);
+        flow_point_pre<T, ENT, DT, POLY, UPW, LAY>(d, rho1, P, wf, keep, &u2c, LAY ? P.zlay + 4L * q : nullptr);
       }
     } else if constexpr (MAG) {
       Derivs<T> d;
@@ -764,7 +765,7 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
           if constexpr (DT) fo.advec_va2 = xi[XVA * TL::NT + t];
         }
         double dt1 = 0.0;
-        flow_point_post<T, MAG, ENT, DT>(keep, fo, P, wf, &dt1);
+        flow_point_post<T, MAG, ENT, DT, LAY>(keep, fo, P, wf, &dt1);
         if constexpr (G::UPDATE) {
           double fo3[NF];
 #pragma unroll
@@ -822,7 +823,8 @@ __device__ __forceinline__ void zsplit_column(const KArgs& a, const Phys& P, int
 // One box: block b marches tile column wid(b) (XCD-aware: consecutive work
 // items, neighbouring tiles that share halo rims, go to the same XCD, whose
 // L2 then serves the rims).
-template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, int UPW = 0>
+// LAY: the samples/conv-slab layer physics (Phys::zlay row of each plane).
+template <typename T, bool MAG, bool ENT, bool POLY, int ST, class TL, int UPW = 0, bool LAY = false>
 __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys P, ZPlan zp) {
   constexpr bool PAIRS = zs_pairs<T, MAG, ENT, POLY, ST, TL, false>();
   constexpr int RING = zs_ring_slots<T, MAG, ENT, ST, TL, PAIRS>();
@@ -837,8 +839,8 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit(KArgs a, Phys 
   double red[NRED];
 #pragma unroll
   for (int q = 0; q < NRED; ++q) red[q] = 0.0;
-  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING, UPW>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz,
-                                                            zp.lz, ring, xch, red);
+  zsplit_column<T, MAG, ENT, POLY, ST, TL, PAIRS, RING, UPW, LAY>(a, P, txt * TL::TX, tyt * TL::TY, chunk * zp.lz,
+                                                                 zp.lz, ring, xch, red);
   if constexpr (StageT<ST>::DIAG) {
     if (a.partials) block_reduce_store<zs_threads<MAG>()>(red, a.partials, (long)blockIdx.x);
   }
@@ -890,14 +892,18 @@ __global__ void __launch_bounds__(zs_threads<MAG>()) k_rhs_zsplit_ens(const EnsB
   }
 }
 
-template <typename T, bool MAG, bool ENT, int ST, class TL, int UPW = 0>
+template <typename T, bool MAG, bool ENT, int ST, class TL, int UPW = 0, bool LAY = false>
 void launch_zsplit(const KArgs& a, const Phys& P, const ZPlan& zp, hipStream_t s) {
-  if (!ENT && P.polytropic)
+  if constexpr (LAY) {  // entropy runs only (split_lay_ok)
+    hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST, TL, UPW, true>), dim3((unsigned)zp.nblocks),
+                       dim3(zs_threads<MAG>()), 0, s, a, P, zp);
+  } else if (!ENT && P.polytropic) {
     hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, false, true, ST, TL, UPW>), dim3((unsigned)zp.nblocks),
                        dim3(zs_threads<MAG>()), 0, s, a, P, zp);
-  else
+  } else {
     hipLaunchKernelGGL((k_rhs_zsplit<T, MAG, ENT, false, ST, TL, UPW>), dim3((unsigned)zp.nblocks),
                        dim3(zs_threads<MAG>()), 0, s, a, P, zp);
+  }
 }
 
 }  // namespace pcmx

@@ -1,5 +1,6 @@
 """Parity of the samples/conv-slab layer physics on the HIP path (direct
-kernel) with the CPU oracle, through the libpcmx C ABI: Coriolis force
+kernel, and the split kernel's entropy variant with lnrho + ss upwinding)
+with the CPU oracle, through the libpcmx C ABI: Coriolis force
 (coriolis_cartesian, hydro.f90:6339-6413), K-profile heat conduction
 (calc_heatcond, entropy.f90:5913-6151), heating and cooling
 (get_heat_cool_gravz, :6761-6828) and the entropy boundary rules 'c1'
@@ -44,12 +45,13 @@ def conv_state(n, seed):
 
 
 def both(n, bcz=BCZ, omega=0.1, theta=0.0, lay_kw=None, ldt=True, dt=2e-4, steps=1, seed=3,
-         precision=pcmx.PREC_FP64, lupw=(False, True, True), gravz=-1.0):
+         precision=pcmx.PREC_FP64, lupw=(False, True, True), gravz=-1.0, kernel=0):
     lay_kw = dict(lay_kw or {})
     ss_rule = bcz[4] if len(bcz) > 4 else "s"
     kw = dict(lentropy=True, nu=4e-3, gamma=GAMMA, cs0=CS0, cdt=0.4, cdtv=0.3, L=L, gravz=gravz, bcz=bcz,
               omega=omega, theta=theta, z0=Z0, lupw_uu=lupw[0], lupw_lnrho=lupw[1], lupw_ss=lupw[2])
-    gp = pcmx.make_params(n, ldt=ldt, kernel=0, precision=precision, layers=layers(pcmx, ss_rule, **lay_kw), **kw)
+    gp = pcmx.make_params(n, ldt=ldt, kernel=kernel, precision=precision, layers=layers(pcmx, ss_rule, **lay_kw),
+                          **kw)
     op = O.make_params(n, dt=0.0 if ldt else dt, layers=layers(O, ss_rule, **lay_kw), **kw)
     f0 = conv_state(n, seed)
     g, t, gdt = pcmx.advance(f0, gp, steps, dt)
@@ -57,11 +59,14 @@ def both(n, bcz=BCZ, omega=0.1, theta=0.0, lay_kw=None, ldt=True, dt=2e-4, steps
     return f0, g, o, (t, gdt, to, odt)
 
 
-def test_conv_slab_one_and_twenty_courant_steps():
+@pytest.mark.parametrize("kernel", [1, 3])
+def test_conv_slab_one_and_twenty_courant_steps(kernel):
+    """kernel 3: the split kernel's entropy variant with lnrho + ss upwinding
+    and the layer terms (32x8 tiles); kernel 1 the direct kernel."""
     n = (32, 16, 24)
     for steps, tol in ((1, FP64_1STEP), (20, FP64_100STEP)):
-        announce(f"test_conv_slab_one_and_twenty_courant_steps n={n} steps={steps}")
-        f0, g, o, (t, gdt, to, odt) = both(n, steps=steps)
+        announce(f"test_conv_slab_one_and_twenty_courant_steps[kernel {kernel}] n={n} steps={steps}")
+        f0, g, o, (t, gdt, to, odt) = both(n, steps=steps, kernel=kernel)
         err = rel_linf(g, o)
         announce(f"  rel L-inf per variable: {err}  dt gpu {gdt!r} oracle {odt!r}")
         assert np.all(err <= tol), err
@@ -117,10 +122,11 @@ def test_isothermal_rotation_with_split_kernel_masks():
     assert np.all(err <= FP64_1STEP * 10), err
 
 
-def test_mixed_form_hundred_steps():
+@pytest.mark.parametrize("kernel", [1, 3])
+def test_mixed_form_hundred_steps(kernel):
     n = (32, 16, 24)
-    announce(f"test_mixed_form_hundred_steps n={n}")
-    f0, g, o, _ = both(n, steps=100, ldt=False, dt=1e-4, precision=pcmx.PREC_MIXED)
+    announce(f"test_mixed_form_hundred_steps[kernel {kernel}] n={n}")
+    f0, g, o, _ = both(n, steps=100, ldt=False, dt=1e-4, precision=pcmx.PREC_MIXED, kernel=kernel)
     err = rel_linf(g, o)
     announce(f"  rel L-inf per variable: {err}")
     assert np.all(err <= MIXED_100STEP), err
@@ -153,8 +159,11 @@ def test_rejected_layer_configurations():
     announce("test_rejected_layer_configurations")
     n = (32, 8, 16)
     kw = dict(lentropy=True, gamma=GAMMA, L=L, bcz=BCZ, z0=Z0)
-    with pytest.raises(pcmx.PcmxError, match="kernel 1"):
+    with pytest.raises(pcmx.PcmxError, match="kernel 1"):    # the split kernel needs lnrho + ss upwinding
         pcmx.initialize_gpu(None, pcmx.make_params(n, kernel=3, layers=layers(pcmx), **kw), nt=0)
+    with pytest.raises(pcmx.PcmxError, match="32x8 tiles"):  # and 32x8 tiles
+        pcmx.initialize_gpu(None, pcmx.make_params((16, 16, 16), kernel=3, layers=layers(pcmx), lupw_lnrho=True,
+                                                   lupw_ss=True, **kw), nt=0)
     no_k = layers(pcmx)
     no_k["lheatc_kprof"] = 0
     with pytest.raises(pcmx.PcmxError, match="'c1' needs"):
