@@ -68,7 +68,8 @@ contains
 !  device state and loads f (gpu_astaroth.f90:192-245).
 !
       use SharedVariables, only: get_shared_variable
-      use EquationOfState, only: rho0
+      use EquationOfState, only: rho0, eos_pushpars2c => pushpars2c
+      use Gravity, only: gravity_pushpars2c => pushpars2c
       use Hydro, only: hydro_pushpars2c => pushpars2c
       use Density, only: density_pushpars2c => pushpars2c
       use Energy, only: energy_pushpars2c => pushpars2c
@@ -80,7 +81,9 @@ contains
       real, dimension(:), pointer :: gz_p
       real(c_double), dimension(3) :: best
       integer(KIND=ikind8), dimension(500) :: p_par   ! the largest module list (entropy.f90:8672)
-      logical, pointer :: lupw
+      logical, pointer :: lupw, lflag
+      real, pointer :: rp
+      integer, pointer :: ip
       character(len=512) :: str
       integer :: j, k
 !
@@ -170,23 +173,82 @@ contains
             case ('s'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_S; else; par%bcz_top(j) = PCMX_BC_S; endif
             case ('a'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_A; else; par%bcz_top(j) = PCMX_BC_A; endif
             case ('a2'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_A2; else; par%bcz_top(j) = PCMX_BC_A2; endif
+            case ('c1'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_C1; else; par%bcz_top(j) = PCMX_BC_C1; endif
+            case ('cT'); if (k == 1) then; par%bcz_bot(j) = PCMX_BC_CT; else; par%bcz_top(j) = PCMX_BC_CT; endif
             case default
               call fatal_error('initialize_GPU', 'bcz='''//trim(bcz12(j,k))// &
-                  ''' is not provided on the GPU (s, a, a2)')
+                  ''' is not provided on the GPU (s, a, a2; c1, cT on ss)')
             endselect
           enddo
         enddo
       endif
 !
-!  The layer physics of samples/conv-slab (rotation, K-profile conduction,
-!  heating and cooling, bcz 'c1' / 'cT') is not mapped from this module's
-!  reference modules: those runs keep the CPU path.
+!  The layer physics of samples/conv-slab.  Rotation: Omega and theta of
+!  Cdata when the Coriolis force is on (hydro.f90:9380 p_par(15) =
+!  lcoriolis_force, which initialize_hydro ands with lrotation).  K-profile
+!  conduction, heating and cooling, and the constants of 'c1' / 'cT', through
+!  the Energy (entropy.f90:8667-), EquationOfState (eos_idealgas.f90:4598-4599)
+!  and Gravity (gravity_simple.f90:1603-1604) pushpars2c lists.
 !
       par%omega = 0.; par%theta = 0.; par%lheatc_kprof = 0; par%cooling_profile = 0
       par%hcond0 = 0.; par%hcond1 = 0.; par%hcond2 = 0.; par%widthss = 0.; par%z1 = 0.; par%z2 = 0.
       par%luminosity = 0.; par%wheat = 0.; par%cool = 0.; par%wcool = 0.; par%cs2cool = 0.
       par%fbot = 0.; par%ftop = 0.; par%cs2bot = 0.; par%cs2top = 0.
       par%z0 = xyz0(3); par%lxyz = Lxyz
+      p_par = 0
+      call hydro_pushpars2c(p_par)
+      call c_f_pointer(transfer(p_par(15), c_null_ptr), lflag)
+      if (lflag .and. Omega /= 0.) then
+        par%omega = Omega; par%theta = theta
+      endif
+      if (lentropy) then
+        p_par = 0
+        call energy_pushpars2c(p_par)
+        call c_f_pointer(transfer(p_par(116), c_null_ptr), lflag)          ! lheatc_kprof
+        if (lflag) then
+          if (.not. lgrav) call fatal_error('initialize_GPU', 'K-profile conduction on the GPU needs gravity')
+          par%lheatc_kprof = 1
+          call c_f_pointer(transfer(p_par(72), c_null_ptr), rp); par%hcond0 = rp
+          call c_f_pointer(transfer(p_par(139), c_null_ptr), lflag)        ! lmultilayer
+          if (lflag) then
+            call c_f_pointer(transfer(p_par(73), c_null_ptr), rp); par%hcond1 = rp
+            call c_f_pointer(transfer(p_par(76), c_null_ptr), rp); par%hcond2 = rp
+          else
+            par%hcond1 = 1.; par%hcond2 = 1.                               ! hcond = hcond0
+          endif
+          call c_f_pointer(transfer(p_par(31), c_null_ptr), rp); par%widthss = rp
+          call c_f_pointer(transfer(p_par(16), c_null_ptr), rp); par%fbot = rp
+          call c_f_pointer(transfer(p_par(17), c_null_ptr), rp); par%ftop = rp
+        endif
+        call c_f_pointer(transfer(p_par(167), c_null_ptr), lflag)          ! lcalc_heat_cool
+        if (lflag) then
+          call c_f_pointer(transfer(p_par(133), c_null_ptr), lflag)        ! lcooling_to_cs2cool
+          if (.not. lflag) call fatal_error('initialize_GPU', 'cooling on the GPU needs lcooling_to_cs2cool')
+          call c_f_pointer(transfer(p_par(192), c_null_ptr), ip)           ! enum_cooling_profile
+          if (ip == enum_gaussian_string) then
+            par%cooling_profile = 0
+          elseif (ip == enum_step_string) then
+            par%cooling_profile = 1
+          else
+            call fatal_error('initialize_GPU', 'cooling_profile is not gaussian or step')
+          endif
+          call c_f_pointer(transfer(p_par(34), c_null_ptr), rp); par%luminosity = rp
+          call c_f_pointer(transfer(p_par(35), c_null_ptr), rp); par%wheat = rp
+          call c_f_pointer(transfer(p_par(36), c_null_ptr), rp); par%cool = rp
+          call c_f_pointer(transfer(p_par(43), c_null_ptr), rp); par%wcool = rp
+          call c_f_pointer(transfer(p_par(53), c_null_ptr), rp); par%cs2cool = rp
+        endif
+        p_par = 0
+        call eos_pushpars2c(p_par)
+        call c_f_pointer(transfer(p_par(10), c_null_ptr), rp); par%cs2bot = rp
+        call c_f_pointer(transfer(p_par(11), c_null_ptr), rp); par%cs2top = rp
+      endif
+      if (lgrav .and. par%lheatc_kprof == 1) then
+        p_par = 0
+        call gravity_pushpars2c(p_par)
+        call c_f_pointer(transfer(p_par(23), c_null_ptr), rp); par%z1 = rp
+        call c_f_pointer(transfer(p_par(24), c_null_ptr), rp); par%z2 = rp
+      endif
 !
       call pcmx_check(pcmx_initialize(f, size(f,4), par, real(t, c_double), 0), 'initialize_GPU')
       if (nt > 0) call load_farray_to_GPU(f)

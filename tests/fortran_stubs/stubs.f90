@@ -10,6 +10,7 @@ module Cparam
   logical, parameter :: lhydro=.true., ldensity=.true., lmagnetic=.true., lentropy=.false.
   logical, parameter :: ltemperature=.false., lthermal_energy=.false., lforcing=.true., lgrav=.true.
   integer, parameter :: mcom=mvar, bclen=3
+  integer, parameter :: enum_step_string=48, enum_gaussian_string=277   ! cparam_enum.h
 end module Cparam
 module Cdata
   use Cparam
@@ -21,6 +22,7 @@ module Cdata
   logical, dimension(3) :: lperi=.true., lequidist=.true.
   integer :: iux=1, ilnrho=4, iss=0, iax=5, itorder=3, nt=10
   real, dimension(3) :: xyz0=-3.14159, Lxyz=6.28318
+  real :: Omega=0., theta=0.
   integer :: n1=4
   character(len=bclen), dimension(mcom,2) :: bcz12=''
 end module Cdata
@@ -62,7 +64,18 @@ end module SharedVariables
 module EquationOfState
   implicit none
   real :: rho0=1.
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(200) :: p_par
+  end subroutine
 end module EquationOfState
+module Gravity
+  implicit none
+contains
+  subroutine pushpars2c(p_par)
+    integer(kind=selected_int_kind(14)), dimension(50) :: p_par
+  end subroutine
+end module Gravity
 module Sub
   implicit none
 contains
