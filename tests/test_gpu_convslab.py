@@ -73,10 +73,11 @@ def test_conv_slab_one_and_twenty_courant_steps(kernel):
         assert gdt == pytest.approx(odt, rel=FP64_1STEP) and t == pytest.approx(to, rel=1e-12)
 
 
+@pytest.mark.parametrize("kernel", [1, 3])
 @pytest.mark.parametrize("piece", ["rotation60", "kprof", "cool_step", "heating", "c1_top_ct_bottom"])
-def test_each_piece_one_and_two_steps(piece):
+def test_each_piece_one_and_two_steps(piece, kernel):
     n = (32, 8, 16)
-    announce(f"test_each_piece_one_and_two_steps[{piece}] n={n}")
+    announce(f"test_each_piece_one_and_two_steps[{piece}, kernel {kernel}] n={n}")
     kw = dict(omega=0.0, lay_kw=dict(hcond0=0.0, cool=0.0, cs2cool=0.0, kprof=False), bcz=["s", "s", "a", "a2", "a2"])
     if piece == "rotation60":
         kw.update(omega=0.23, theta=60.0)
@@ -95,7 +96,7 @@ def test_each_piece_one_and_two_steps(piece):
         kw.update(bcz=["s", "s", "a", "a2", "cT:c1"], omega=0.1)
         kw["lay_kw"] = dict(cs2bot=0.5, ftop=0.004)
     for steps, tol in ((1, FP64_1STEP), (2, 10 * FP64_1STEP)):
-        f0, g, o, _ = both(n, steps=steps, **kw)
+        f0, g, o, _ = both(n, steps=steps, kernel=kernel, **kw)
         err = rel_linf(g, o)
         announce(f"  {steps} steps: rel L-inf per variable: {err}")
         assert np.all(err <= tol), err
@@ -105,6 +106,30 @@ def test_each_piece_one_and_two_steps(piece):
                                        lupw_ss=True, dt=kw.get("dt", 0.0) if not kw.get("ldt", True) else 0.0),
                      2)[0]
     assert rel_linf(o, base).max() > 1e-9
+
+
+@pytest.mark.parametrize("kernel", [1, 3])
+def test_magnetic_conv_slab(kernel):
+    """The conv-slab physics with a magnetic field (bcz a, a, s for A):
+    Lorentz force and ohmic heating beside the layer terms."""
+    n = (32, 16, 24)
+    bcz = BCZ + ["a", "a", "s"]
+    kw = dict(lentropy=True, lmagnetic=True, nu=4e-3, eta=3e-3, gamma=GAMMA, cs0=CS0, cdt=0.4, cdtv=0.3, L=L,
+              gravz=-1.0, bcz=bcz, omega=0.1, z0=Z0, lupw_lnrho=True, lupw_ss=True)
+    f0 = pcmx_ics.config_state("shells", n, 9, lmagnetic=True, lentropy=True)
+    ref = conv_state(n, 9)
+    f0[:5] = ref
+    f0[5:] *= 0.3
+    for steps, tol in ((1, FP64_1STEP), (10, FP64_100STEP)):
+        announce(f"test_magnetic_conv_slab[kernel {kernel}] n={n} steps={steps}")
+        gp = pcmx.make_params(n, ldt=True, kernel=kernel, layers=layers(pcmx), **kw)
+        op = O.make_params(n, dt=0.0, layers=layers(O), **kw)
+        g, t, gdt = pcmx.advance(f0, gp, steps, 0.0)
+        o, to, odt, _ = O.advance(f0, op, steps)
+        err = rel_linf(g, o)
+        announce(f"  rel L-inf per variable: {err}")
+        assert np.all(err <= tol), err
+        assert gdt == pytest.approx(odt, rel=FP64_1STEP)
 
 
 def test_isothermal_rotation_with_split_kernel_masks():
