@@ -374,23 +374,19 @@ def run_convslab(n, steps, warmup, tune=True):
     top cooling (cool = 15, wcool = 0.2, cs2cool = 0.333333), bcz = s, s, a,
     a2, c1:cT, lupw_lnrho = lupw_ss = T, Courant steps with cdt = 0.4,
     cdtv = 0.3, gamma = 1.6666666, cs0 = 0.57735 (samples/conv-slab/start.in,
-    run.in).  Initial state: lnrho = -0.8 (z - z0) and ss = -0.1 (z - z0)
-    plus noise (seed 3), velocity noise; not the reference's piecewise
-    polytrope.  Runs on the split kernel (its entropy variant with lnrho + ss
-    upwinding and the layer terms), tuned first."""
+    run.in), from the sample's own initial state (piecewise polytrope and
+    'up-down' u_z, pcmx_ics.conv_slab_state).  Runs on the split kernel (its
+    entropy variant with lnrho + ss upwinding and the layer terms), tuned
+    first."""
     dims = (n, n, 2 * n)
-    interior = pcmx_ics.config_state("noise", dims, 3, lentropy=True)
-    z = -0.68 + 2.0 / (2 * n - 1) * np.arange(2 * n)[:, None, None]
-    interior[:3] *= 0.5
-    interior[3] += -0.8 * (z + 0.68)
-    interior[4] = 0.05 * interior[4] - 0.1 * (z + 0.68)
+    interior = pcmx_ics.conv_slab_state(n, 2 * n)
     f = pcmx.farray_from_interior(interior)
     del interior
     lay = pcmx.layer_defaults(1.6666666, 1.0, -1.0, "c1:cT", hcond0=8e-3, mpoly0=1.0, mpoly1=3.0, mpoly2=0.0,
                               widthss=0.05, z1=0.0, z2=1.0, cool=15.0, wcool=0.2, cs2cool=0.333333)
     desc = (f"samples/conv-slab physics {n}x{n}x{2 * n}, 5 FP64 vars: gravz=-1, Omega=0.1, nu=4e-3, K-profile "
             "(hcond0=8e-3, mpoly 1/3/0), cooling (cool=15, wcool=0.2), bcz s,s,a,a2,c1:cT, "
-            "lupw_lnrho=lupw_ss=T, Courant dt (cdt=0.4, cdtv=0.3); stratified noise state (seed 3)")
+            "lupw_lnrho=lupw_ss=T, Courant dt (cdt=0.4, cdtv=0.3); the sample's piecewise-polytrope initial state")
     log(f"bench: conv-slab physics: {desc}")
     out = {"workload": desc, "unit": "Mcell-updates/s", "grid": list(dims)}
     for name, prec in (("fp64", pcmx.PREC_FP64), ("mixed", pcmx.PREC_MIXED)):

@@ -139,3 +139,88 @@ def config_state(kind, n, seed, lmagnetic=False, lentropy=False):
     else:
         raise ValueError(kind)
     return f
+
+
+def _step(x, x0, w):
+    """step (sub.f90:4975-4987), tini = 5*tiny."""
+    return 0.5 * (1 + np.tanh((x - x0) / (w + 5 * 2.2250738585072014e-308)))
+
+
+def _polytrope_layer(f, z, kind, mpoly, zint, zbot, zblend, isoth, cs2int, vint, gamma, gravz, width,
+                     cp=1.0, fac_cs=None):
+    """One layer of polytropic_lnrho_z (density.f90:2111-2177, kind 'lnrho')
+    or polytropic_ss_z (entropy.f90:2186-2260, kind 'ss'), called from the
+    top layer down; f is the (nz,) profile, blended in below zblend with
+    step(z, zblend, width).  Returns the new profile, cs2int and the value at
+    the layer bottom."""
+    epsi = 5 * np.finfo(np.float64).eps
+    gamma_m1, cp1 = gamma - 1.0, 1.0 / cp
+    stp = _step(z, zblend, width)
+    if isoth:
+        beta1 = 0.0
+        if kind == "lnrho":
+            tmp = vint + gamma * gravz / cs2int * (z - zint)
+        else:
+            tmp = vint - gamma_m1 * gravz * (z - zint) / cs2int / cp1
+    else:
+        beta1 = gamma * gravz / (mpoly + 1)
+        t = np.maximum(1.0 + beta1 * (z - zint) / cs2int, epsi)
+        tmp = vint + mpoly * np.log(t) if kind == "lnrho" else vint + (1 - mpoly * gamma_m1) / gamma * np.log(t) / cp1
+    f = stp * f + (1 - stp) * tmp
+    if isoth:
+        if kind == "lnrho":
+            vint = vint + gamma * gravz / cs2int * (zbot - zint)
+        else:
+            vint = vint - gamma_m1 * gravz * (zbot - zint) / cs2int / cp1
+            if fac_cs is not None:
+                vint = vint + 2 * np.log(fac_cs) / gamma / cp1
+    elif kind == "lnrho":
+        vint = vint + mpoly * np.log(1 + beta1 * (zbot - zint) / cs2int)
+    else:
+        vint = vint + (1 - mpoly * gamma_m1) / gamma * np.log(1 + beta1 * (zbot - zint) / cs2int) / cp1
+    cs2int = fac_cs ** 2 * cs2int if (isoth and fac_cs is not None) else cs2int + beta1 * (zbot - zint)
+    return f, cs2int, vint
+
+
+def conv_slab_state(n=32, nz=None):
+    """The initial state of samples/conv-slab (start.in) on an n x n x nz grid
+    (default nz = n): the 'piecew-poly' stratification of lnrho (density.f90:
+    1578-1605) and ss (entropy.f90:1971-1993) -- an isothermal top layer
+    (isothtop = 1) above z2 = 1, a polytrope of index mpoly0 = 1 between
+    z1 = 0 and z2, mpoly1 = 3 below, referred to zref = 1.32 where cs = cs0 =
+    0.57735, rho = 1, ss = 0, blended with width 0.05 -- and the 'up-down'
+    u_z (hydro.f90:2633-2647, ampluu = 0.1, widthuu = 0.1, about z1 = 0).
+    Box xyz0 = (-0.5, -0.5, -0.68), Lxyz = (1, 1, 2), periodic in x and y
+    (cell centres), z points from -0.68 to 1.32.  Returns (5, nz, n, n)."""
+    nz = n if nz is None else nz
+    gamma, gravz, cs0 = 1.6666666, -1.0, 0.57735
+    x0, y0, z0, Lx, Ly, Lz = -0.5, -0.5, -0.68, 1.0, 1.0, 2.0
+    zref, z1, z2, ztop = 1.32, 0.0, 1.0, z0 + Lz
+    mpoly0, mpoly1, mpoly2 = 1.0, 3.0, 0.0
+    x = x0 + (np.arange(n) + 0.5) * (Lx / n)
+    y = y0 + (np.arange(n) + 0.5) * (Ly / n)
+    z = z0 + np.arange(nz) * (Lz / (nz - 1))
+    # lnrho: top (isothermal), unstable, stable layer
+    lnrho = np.zeros(nz)
+    cs2int, lint = cs0 ** 2, 0.0
+    lnrho, cs2int, lint = _polytrope_layer(lnrho, z, "lnrho", mpoly2, zref, z2, ztop + Lz, 1, cs2int, lint,
+                                           gamma, gravz, 0.05, fac_cs=1.0)
+    lnrho, cs2int, lint = _polytrope_layer(lnrho, z, "lnrho", mpoly0, z2, z1, z2, 0, cs2int, lint, gamma, gravz, 0.05)
+    lnrho, cs2int, lint = _polytrope_layer(lnrho, z, "lnrho", mpoly1, z1, z0, z1, 0, cs2int, lint, gamma, gravz, 0.05)
+    ss = np.zeros(nz)
+    cs2int, sint = cs0 ** 2, 0.0
+    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly2, zref, z2, z0 + 2 * Lz, 1, cs2int, sint, gamma, gravz,
+                                        0.05, fac_cs=1.0)
+    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly0, z2, z1, z2, 0, cs2int, sint, gamma, gravz, 0.05)
+    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly1, z1, z0, z1, 0, cs2int, sint, gamma, gravz, 0.05)
+    f = np.zeros((5, nz, n, n))
+    f[3] = lnrho[:, None, None]
+    f[4] = ss[:, None, None]
+    w = 0.1
+    prof = 0.1 * np.exp(-0.5 * (z - z1) ** 2 / w ** 2)[:, None, None]
+    X, Y = x[None, None, :], y[None, :, None]
+    t1 = np.sqrt((X - (x0 + 0.3 * Lx)) ** 2 + (Y - (y0 + 0.3 * Ly)) ** 2)
+    t2 = np.sqrt((X - (x0 + 0.5 * Lx)) ** 2 + (Y - (y0 + 0.8 * Ly)) ** 2)
+    f[2] = prof * np.exp(-0.5 * (t1 ** 2) / w ** 2)
+    f[2] = f[2] - 0.7 * prof * np.exp(-0.5 * (t2 ** 2) / w ** 2)
+    return f
