@@ -398,6 +398,27 @@ def run_convslab(n, steps, warmup, tune=True):
                      "kernel_ms": round(r["kernel_ms"], 4), "roofline_frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
                      "dt_last": r["dt"], "t": r["t"], "urms": r["urms"], "plan": r["plan"]}
         log(f"bench: conv-slab {name}: {out[name]}")
+    # the oracle (the CPU restatement of the reference's pencil loop) on the
+    # same physics and state recipe, a bounded 64 x 64 x 128 sample, 2 steps
+    try:
+        O = oracle()
+        cdims = (64, 64, 128)
+        clay = O.layer_defaults(1.6666666, 1.0, -1.0, "c1:cT", hcond0=8e-3, mpoly0=1.0, mpoly1=3.0, mpoly2=0.0,
+                                widthss=0.05, z1=0.0, z2=1.0, cool=15.0, wcool=0.2, cs2cool=0.333333)
+        op = O.make_params(cdims, lentropy=True, nu=4e-3, gamma=1.6666666, cs0=0.57735, cdt=0.4, cdtv=0.3, dt=0.0,
+                           L=(1.0, 1.0, 2.0), gravz=-1.0, bcz=CONVSLAB_BC, omega=0.1, z0=-0.68, lupw_lnrho=True,
+                           lupw_ss=True, layers=clay)
+        cstate = pcmx_ics.conv_slab_state(64, 128)
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        O.advance(cstate, op, 2, nthreads=threads)
+        csec = time.perf_counter() - t0
+        out["cpu_oracle"] = {"value": round(64 * 64 * 128 * 2 / csec / 1e6, 2), "cores": threads,
+                             "sample": "64x64x128, 2 Courant steps, same physics and state recipe"}
+        log(f"bench: conv-slab oracle on the host: {out['cpu_oracle']}")
+    except (ImportError, OSError) as exc:
+        out["cpu_oracle"] = None
+        log(f"bench: conv-slab oracle not available: {exc}")
     return out
 
 

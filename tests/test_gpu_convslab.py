@@ -132,6 +132,20 @@ def test_magnetic_conv_slab(kernel):
         assert gdt == pytest.approx(odt, rel=FP64_1STEP)
 
 
+@pytest.mark.parametrize("kernel", [1, 3])
+def test_entropy_rules_read_a_zeroed_lnrho_boundary(kernel):
+    """lnrho with 'a' (its boundary value zeroed before the entropy rules
+    read it, as the variable loop of boundconds_z orders them) under
+    'c1:cT' and 'cT:c1' for ss: one step."""
+    n = (32, 8, 16)
+    for ss_rule, lay_kw in (("c1:cT", {}), ("cT:c1", dict(cs2bot=0.5, ftop=0.004))):
+        announce(f"test_entropy_rules_read_a_zeroed_lnrho_boundary[kernel {kernel}] ss {ss_rule}")
+        f0, g, o, _ = both(n, bcz=["s", "s", "a", "a", ss_rule], kernel=kernel, lay_kw=lay_kw)
+        err = rel_linf(g, o)
+        announce(f"  rel L-inf per variable: {err}")
+        assert np.all(err <= FP64_1STEP), err
+
+
 def test_isothermal_rotation_with_split_kernel_masks():
     """Rotation without entropy on the direct kernel, with uu + lnrho
     upwinding (the published upwinding set)."""
