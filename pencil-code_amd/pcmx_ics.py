@@ -224,3 +224,35 @@ def conv_slab_state(n=32, nz=None):
     f[2] = prof * np.exp(-0.5 * (t1 ** 2) / w ** 2)
     f[2] = f[2] - 0.7 * prof * np.exp(-0.5 * (t2 ** 2) / w ** 2)
     return f
+
+
+def fluxrings_state(n=32):
+    """The initial state of samples/interlocked-fluxrings (start.in): two
+    interlocked magnetic flux rings in A (initial_condition/fluxrings.f90:
+    fluxrings :204-290, norm_ring 'tanh' profile :300-360), flux 1, radius
+    1, width 0.3, ring 1 about the z axis displaced to (0, -0.5, 0), ring 2
+    about the x axis displaced to (0, 0.5, 0), no current (Iring = 0); u = 0,
+    lnrho = 0; periodic box [-pi, pi]^3 (cell centres).  Returns (7, n, n, n)
+    in the isothermal MHD slots uu, lnrho, aa."""
+    epsi = 5 * np.finfo(np.float64).eps
+    x = -np.pi + (np.arange(n) + 0.5) * (2 * np.pi / n)
+    X, Y, Z = x[None, None, :], x[None, :, None], x[:, None, None]
+    f = np.zeros((7, n, n, n))
+    for axis, disp in (((0.0, 0.0, 1.0), (0.0, -0.5, 0.0)), ((1.0, 0.0, 0.0), (0.0, 0.5, 0.0))):
+        fring, R0, width, ampl = 1.0, 1.0, 0.3, 1.0
+        phi = np.arctan2(axis[1], axis[0] + epsi)
+        theta = np.arctan2(np.sqrt(axis[0] ** 2 + axis[1] ** 2) + epsi, axis[2])
+        ct, st, cp, sp = np.cos(theta), np.sin(theta), np.cos(phi), np.sin(phi)
+        dx, dy, dz = X - disp[0], Y - disp[1], Z - disp[2]
+        xx1 = ct * cp * dx + ct * sp * dy - st * dz
+        yy1 = -sp * dx + cp * dy
+        zz1 = st * cp * dx + st * sp * dy + ct * dz
+        tmp = np.sqrt(xx1 ** 2 + yy1 ** 2) - R0
+        v3 = -(fring * 0.5 * (1 + np.tanh(tmp / width)) * 0.5 / width / np.cosh(zz1 / width) ** 2)
+        tmp = 0.0 * 0.5 * (1 + np.tanh((width - np.sqrt(tmp ** 2 + zz1 ** 2)) / width))   # Iring = 0
+        ph = np.arctan2(yy1, xx1)
+        v1, v2 = -tmp * np.sin(ph), tmp * np.cos(ph)
+        f[4] = f[4] + ampl * (+ct * cp * v1 - sp * v2 + st * cp * v3)
+        f[5] = f[5] + ampl * (+ct * sp * v1 + cp * v2 + st * sp * v3)
+        f[6] = f[6] + ampl * (-st * v1 + ct * v3)
+    return f
