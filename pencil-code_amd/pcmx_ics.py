@@ -182,7 +182,7 @@ def _polytrope_layer(f, z, kind, mpoly, zint, zbot, zblend, isoth, cs2int, vint,
     return f, cs2int, vint
 
 
-def conv_slab_state(n=32, nz=None):
+def conv_slab_state(n=32, nz=None, cp=1.0):
     """The initial state of samples/conv-slab (start.in) on an n x n x nz grid
     (default nz = n): the 'piecew-poly' stratification of lnrho (density.f90:
     1578-1605) and ss (entropy.f90:1971-1993) -- an isothermal top layer
@@ -191,7 +191,8 @@ def conv_slab_state(n=32, nz=None):
     0.57735, rho = 1, ss = 0, blended with width 0.05 -- and the 'up-down'
     u_z (hydro.f90:2633-2647, ampluu = 0.1, widthuu = 0.1, about z1 = 0).
     Box xyz0 = (-0.5, -0.5, -0.68), Lxyz = (1, 1, 2), periodic in x and y
-    (cell centres), z points from -0.68 to 1.32.  Returns (5, nz, n, n)."""
+    (cell centres), z points from -0.68 to 1.32.  cp scales ss (the
+    samples/conv-slab_cp_2 variant runs cp = 2).  Returns (5, nz, n, n)."""
     nz = n if nz is None else nz
     gamma, gravz, cs0 = 1.6666666, -1.0, 0.57735
     x0, y0, z0, Lx, Ly, Lz = -0.5, -0.5, -0.68, 1.0, 1.0, 2.0
@@ -210,9 +211,9 @@ def conv_slab_state(n=32, nz=None):
     ss = np.zeros(nz)
     cs2int, sint = cs0 ** 2, 0.0
     ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly2, zref, z2, z0 + 2 * Lz, 1, cs2int, sint, gamma, gravz,
-                                        0.05, fac_cs=1.0)
-    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly0, z2, z1, z2, 0, cs2int, sint, gamma, gravz, 0.05)
-    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly1, z1, z0, z1, 0, cs2int, sint, gamma, gravz, 0.05)
+                                        0.05, cp=cp, fac_cs=1.0)
+    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly0, z2, z1, z2, 0, cs2int, sint, gamma, gravz, 0.05, cp=cp)
+    ss, cs2int, sint = _polytrope_layer(ss, z, "ss", mpoly1, z1, z0, z1, 0, cs2int, sint, gamma, gravz, 0.05, cp=cp)
     f = np.zeros((5, nz, n, n))
     f[3] = lnrho[:, None, None]
     f[4] = ss[:, None, None]

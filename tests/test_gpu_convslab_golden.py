@@ -11,18 +11,21 @@ import oracle as O
 import pcmx
 import pcmx_ics
 from test_gpu_parity import announce
-from test_oracle_convslab_golden import CONV_SLAB_KW, check_row, conv_slab_layers, golden_rows, means_after_bcs
+from test_oracle_convslab_golden import (CONV_SLAB_KW, VARIANTS, check_row, conv_slab_layers, golden_rows,
+                                         means_after_bcs)
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
-def test_conv_slab_sample_reproduces_the_reference_run(kernel):
-    announce(f"test_conv_slab_sample_reproduces_the_reference_run[kernel {kernel}]")
-    rows = golden_rows()
-    gp = pcmx.make_params((32, 32, 32), ldt=True, kernel=kernel, layers=conv_slab_layers(pcmx), **CONV_SLAB_KW)
-    op = O.make_params((32, 32, 32), dt=0.0, layers=conv_slab_layers(O), **CONV_SLAB_KW)
-    f0 = pcmx_ics.conv_slab_state(32)
+@pytest.mark.parametrize("kernel,variant", [(0, "cp1"), (1, "cp1"), (0, "cp2")])
+def test_conv_slab_sample_reproduces_the_reference_run(kernel, variant):
+    announce(f"test_conv_slab_sample_reproduces_the_reference_run[kernel {kernel}, {variant}]")
+    rows = golden_rows(variant)
+    cp = VARIANTS[variant][1]
+    gp = pcmx.make_params((32, 32, 32), ldt=True, kernel=kernel, cp=cp, layers=conv_slab_layers(pcmx, variant),
+                          **CONV_SLAB_KW)
+    op = O.make_params((32, 32, 32), dt=0.0, cp=cp, layers=conv_slab_layers(O, variant), **CONV_SLAB_KW)
+    f0 = pcmx_ics.conv_slab_state(32, cp=cp)
     f = pcmx.farray_from_interior(f0)
     pcmx.initialize_gpu(f, gp, nt=1)
     try:

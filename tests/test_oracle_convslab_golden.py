@@ -20,13 +20,20 @@ HERE = os.path.dirname(__file__)
 RTOL = {"t": 1e-13, "dt": 1e-13, "urms": 2e-12, "umax": 2e-12, "rhom": 1e-13, "ssm": 1e-13}
 
 
-def golden_rows():
-    with open(os.path.join(HERE, "golden", "conv_slab_reference_double.json")) as fh:
+# (fixture, cp, hcond0): samples/conv-slab and samples/conv-slab_cp_2 (cp = 2,
+# hcond0 doubled; its file prints mean(ss)/cp, the same physical run)
+VARIANTS = {"cp1": ("conv_slab_reference_double.json", 1.0, 8e-3),
+            "cp2": ("conv_slab_cp2_reference_double.json", 2.0, 1.6e-2)}
+
+
+def golden_rows(variant="cp1"):
+    with open(os.path.join(HERE, "golden", VARIANTS[variant][0])) as fh:
         return json.load(fh)["rows"]
 
 
-def conv_slab_layers(mod):
-    return mod.layer_defaults(1.6666666, 1.0, -1.0, "c1:cT", hcond0=8e-3, mpoly0=1.0, mpoly1=3.0, mpoly2=0.0,
+def conv_slab_layers(mod, variant="cp1"):
+    _, cp, hcond0 = VARIANTS[variant]
+    return mod.layer_defaults(1.6666666, cp, -1.0, "c1:cT", hcond0=hcond0, mpoly0=1.0, mpoly1=3.0, mpoly2=0.0,
                               widthss=0.05, z1=0.0, z2=1.0, cool=15.0, wcool=0.2, cs2cool=0.333333)
 
 
@@ -41,18 +48,20 @@ def check_row(ours, ref):
 
 
 def means_after_bcs(interior, p):
-    """rhom and ssm as the reference's diagnostics see them: after the
-    boundary conditions of the pde call ('cT' sets ss on the top plane)."""
+    """rhom and mean(ss)/cp as the reference's diagnostics see them: after
+    the boundary conditions of the pde call ('cT' sets ss on the top plane)."""
     g = O.with_ghosts(interior)
     O.boundconds(g, p)
     gi = O.interior(g)
-    return float(np.exp(gi[3]).mean()), float(gi[4].mean())
+    return float(np.exp(gi[3]).mean()), float(gi[4].mean()) / p.cp
 
 
-def test_oracle_reproduces_the_reference_run():
-    rows = golden_rows()
-    p = O.make_params((32, 32, 32), dt=0.0, layers=conv_slab_layers(O), **CONV_SLAB_KW)
-    f = pcmx_ics.conv_slab_state(32)
+@pytest.mark.parametrize("variant", ["cp1", "cp2"])
+def test_oracle_reproduces_the_reference_run(variant):
+    rows = golden_rows(variant)
+    cp = VARIANTS[variant][1]
+    p = O.make_params((32, 32, 32), dt=0.0, cp=cp, layers=conv_slab_layers(O, variant), **CONV_SLAB_KW)
+    f = pcmx_ics.conv_slab_state(32, cp=cp)
     t = 0.0
     for it in range(41):
         f1, t1, dt, ser = O.advance(f, p, 1, t=t, diag=True)
